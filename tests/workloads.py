@@ -1,0 +1,62 @@
+"""Seeded block-sparse effective-Hamiltonian workloads shared by tests and the bench.
+
+Bond sectors follow the charge bookkeeping of a finite MPS (site 0 carries
+the total charge, boundary bonds are one-dimensional with zero charge); the
+environment blocks are filled with U[-1, 1) so that every structurally
+allowed block is present.
+"""
+import numpy as np
+
+from oracle.bsten import IN, OUT, BSTensor, Index, qn_add, qn_sub
+from oracle.models import build_mpo, heisenberg_chain, hubbard_cylinder, phys_index
+
+
+def gaussian_bond(qns_site, width, D, nsect_half, qn_len=1):
+    """Sector profile exp(-q^2/(2 width^2)) over charges -K..K (U(1)), scaled to total D."""
+    if qn_len == 1:
+        qs = [(2 * k,) for k in range(-nsect_half, nsect_half + 1)]
+        w = np.array([np.exp(-(q[0] / 2) ** 2 / (2 * width ** 2)) for q in qs])
+    else:
+        qs = [(n, 2 * s) for n in range(-nsect_half, nsect_half + 1) for s in range(-nsect_half, nsect_half + 1)]
+        w = np.array([np.exp(-(q[0] ** 2 + (q[1] / 2) ** 2) / (2 * width ** 2)) for q in qs])
+    degs = np.maximum(1, np.round(w / w.sum() * D)).astype(int)
+    return [(q, int(d)) for q, d in zip(qs, degs)]
+
+
+def synthetic_eff_ham(kind, D, seed, nsect_half=None, width=None, site_model=None):
+    """Random L, Wl, Wr, R, psi at bond dimension ~D.
+
+    kind = "heisenberg": U(1) spin chain MPO (w=5); kind = "hubbard": U(1)xU(1)
+    4-leg cylinder MPO taken from the middle of a 4x16 snake.
+    """
+    rng = np.random.default_rng(seed)
+    if kind == "heisenberg":
+        m = heisenberg_chain(8)
+        mpo = build_mpo(m)
+        Wl, Wr = mpo[3], mpo[4]
+        nsect_half = nsect_half or 12
+        width = width or max(1.0, nsect_half / 2.5)
+        secs = gaussian_bond(m.qns, width, D, nsect_half, 1)
+    else:
+        m = hubbard_cylinder(16, 4) if site_model is None else site_model
+        mpo = build_mpo(m)
+        mid = m.n // 2
+        Wl, Wr = mpo[mid - 1], mpo[mid]
+        nsect_half = nsect_half or 5
+        width = width or 2.0
+        secs = gaussian_bond(m.qns, width, D, nsect_half, 2)
+    zero = tuple(0 for _ in m.qns[0])
+    a_out = Index.make(secs, OUT)        # right bond of the left block
+    a_in = a_out.inverse()
+    b_out = Index.make(secs, OUT)
+    b_in = b_out.inverse()
+    pout = phys_index(m.qns, OUT)
+    L = BSTensor([a_in, Wl.indexes[0].inverse(), a_out], np.float64, div=zero)
+    R = BSTensor([b_out, Wr.indexes[3].inverse(), b_in], np.float64, div=zero)
+    for t in (L, R):
+        for c in t.allowed_blocks(zero):
+            t.blocks[c] = rng.uniform(-1, 1, size=t.blk_shape(c))
+    psi = BSTensor([a_in, pout, pout, b_out], np.float64, div=zero)
+    for c in psi.allowed_blocks(zero):
+        psi.blocks[c] = rng.uniform(-1, 1, size=psi.blk_shape(c))
+    return [L, Wl, Wr, R], psi
