@@ -1,0 +1,175 @@
+/*
+ * qlgpu.h -- C ABI of the MI355X (gfx950) effective-Hamiltonian matvec and
+ * Krylov solvers for two-site DMRG / TDVP over U(1) and U(1)xU(1)
+ * block-sparse tensors.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (QLG_OK == 0).  Nothing calls
+ *     exit() or throws across this boundary; the message of the last failure
+ *     on a context is available from qlg_last_error().
+ *   - All tensor data and Krylov vectors are device pointers owned by the
+ *     caller.  A plan owns its own workspace (intermediates, task lists,
+ *     Krylov basis) and frees it in qlg_plan_destroy().
+ *   - One HIP stream per context; calls on one context are not thread safe.
+ *   - Complex data (QLG_C128) is interleaved (re, im) double pairs.
+ *
+ * Block-sparse tensor descriptor (qlg_bsten): an index is an ordered list of
+ * quantum-number sectors (qn, degeneracy) with a direction; a block is one
+ * sector per axis and is stored dense, row-major, at blk_offset[] elements
+ * from data.  A block may exist only where sum_OUT qn - sum_IN qn equals the
+ * tensor's divergence.  This is the storage model of the reference's
+ * QLTensor (TensorToolkit), so a caller can hand its own block map over
+ * without copying.
+ */
+#ifndef QLGPU_H
+#define QLGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define QLG_OK 0
+#define QLG_ERR_INVALID 1      /* malformed descriptor / argument          */
+#define QLG_ERR_HIP 2          /* a HIP runtime call failed                */
+#define QLG_ERR_ALLOC 3        /* device allocation failed                 */
+#define QLG_ERR_UNSUPPORTED 4  /* valid request this build does not handle */
+#define QLG_ERR_NUMERIC 5      /* tridiagonal eigen-solver did not converge*/
+
+/* ---- element / arithmetic types ---------------------------------------- */
+#define QLG_F64 0   /* double                                            */
+#define QLG_C128 1  /* complex double, interleaved (re, im)              */
+
+/* Arithmetic of the sector GEMMs.  Results, the Krylov recurrence, the
+ * tridiagonal solve and all reductions stay FP64 in every mode. */
+#define QLG_PREC_F64 0     /* FP64 MFMA                                    */
+#define QLG_PREC_BF16X3 2  /* operands split in 3 bf16 terms, 3 products  */
+#define QLG_PREC_BF16X6 3  /* operands split in 3 bf16 terms, 6 products  */
+
+#define QLG_MAX_RANK 8
+#define QLG_MAX_QN_LEN 2
+
+typedef struct qlg_ctx_s *qlg_ctx;
+typedef struct qlg_plan_s *qlg_plan;
+
+typedef struct {
+  int32_t rank;               /* <= QLG_MAX_RANK                            */
+  int32_t qn_len;             /* 1: U(1); 2: U(1)xU(1)                      */
+  const int32_t *nsect;       /* [rank] sectors per axis                    */
+  const int32_t *sect_qn;     /* [sum(nsect) * qn_len], axis-major          */
+  const int64_t *sect_deg;    /* [sum(nsect)] degeneracies                  */
+  const int8_t *dir;          /* [rank] -1 = IN, +1 = OUT                   */
+  int64_t nblocks;
+  const int32_t *blk_coords;  /* [nblocks * rank] sector id per axis        */
+  const int64_t *blk_offset;  /* [nblocks] element offset of each block     */
+  int32_t dtype;              /* QLG_F64 / QLG_C128                         */
+  const void *data;           /* device pointer                             */
+} qlg_bsten;
+
+typedef struct {
+  double error;               /* LanczosParams::error                      */
+  int64_t max_iterations;     /* LanczosParams::max_iterations             */
+} qlg_lanczos_params;
+
+typedef struct {
+  int64_t iters;              /* LanczosRes::iters                          */
+  double gs_eng;              /* LanczosRes::gs_eng                         */
+} qlg_lanczos_result;
+
+/* ---- context ------------------------------------------------------------ */
+/* stream: a hipStream_t, or NULL for the library's own stream. */
+int qlg_ctx_create(int device, void *stream, qlg_ctx *out);
+int qlg_ctx_destroy(qlg_ctx ctx);
+const char *qlg_last_error(qlg_ctx ctx);
+int qlg_ctx_synchronize(qlg_ctx ctx);
+
+/* ---- effective-Hamiltonian plans ----------------------------------------
+ * Replaces the {L, W_l, W_r, R} operand vector and the function pointer
+ * eff_ham_mul_two_site_state of
+ *   include/qlmps/algorithm/vmps/lanczos_vmps_solver_impl.h:114-120, 260-289.
+ * Index orders follow the reference: L[bra, mpo, ket], W[wl, s_in, s_out, wr],
+ * R[bra, mpo, ket], psi[a, s1, s2, b].  The plan pairs quantum-number sectors
+ * once and keeps the GEMM task lists for every later matvec with the same
+ * operators; the operator data must stay alive and unchanged meanwhile.
+ * The Krylov vector layout (block list and offsets) is the closure of psi's
+ * blocks under H_eff; read it with qlg_plan_vector_layout.               */
+int qlg_plan_two_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *Wl,
+                      const qlg_bsten *Wr, const qlg_bsten *R,
+                      const qlg_bsten *psi, int precision, qlg_plan *out);
+/* One-site form, lanczos_vmps_solver_impl.h:291-304 (psi[a, s, b]). */
+int qlg_plan_one_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *W,
+                      const qlg_bsten *R, const qlg_bsten *psi,
+                      int precision, qlg_plan *out);
+int qlg_plan_destroy(qlg_plan plan);
+int qlg_plan_set_precision(qlg_plan plan, int precision);
+/* Algorithmic FLOPs of one matvec: sum of 2*M*N*K (8*M*N*K complex) over the
+ * sector-GEMM pairs of the reference's four-step schedule
+ * (lanczos_vmps_solver_impl.h:268-272).  exec_flops: FLOPs the plan's own
+ * kernels issue (per product term).  Either pointer may be NULL.          */
+int qlg_plan_flops(qlg_plan plan, double *algo_flops, double *exec_flops);
+/* Vector layout: *nblocks, *size (elements); coords [nblocks*rank] and
+ * offsets [nblocks] are filled when non-NULL (blocks in lexicographic order
+ * of their sector coordinates, offsets = prefix sums of block sizes).     */
+int qlg_plan_vector_layout(qlg_plan plan, int64_t *nblocks, int64_t *size,
+                           int32_t *coords, int64_t *offsets);
+/* Workspace bytes held by the plan (intermediates + task lists). */
+int qlg_plan_workspace(qlg_plan plan, int64_t *bytes);
+/* Kernel time of the last matvec per stage (ms, HIP events on the plan's
+ * stream): [0] = first contraction, [1] = MPO stage, [2] = last contraction.
+ * Valid after qlg_plan_enable_timing(plan, 1) and a synchronised matvec.  */
+int qlg_plan_enable_timing(qlg_plan plan, int on);
+int qlg_plan_stage_ms(qlg_plan plan, double *ms3);
+/* FLOPs each stage issues per matvec (per product term), same order. */
+int qlg_plan_stage_flops(qlg_plan plan, double *flops3);
+
+/* out = H_eff * psi, both in the plan's vector layout (device). */
+int qlg_matvec(qlg_plan plan, const void *psi, void *out);
+
+/* ---- Krylov vector kernels (flat vectors of n elements) ------------------
+ * qlten LinearCombine / Normalize / Dag+Contract equivalents,
+ * lanczos_vmps_solver_impl.h:141,155-161,173-183,238.  Reductions are
+ * deterministic (fixed grid, ordered two-pass).  Scalars are host values;
+ * complex scalars are (re, im) pairs.                                      */
+int qlg_dot(qlg_ctx ctx, int dtype, int64_t n, const void *x, const void *y,
+            double *out /* sum x*conj(y); 1 or 2 doubles */);
+int qlg_nrm2(qlg_ctx ctx, int dtype, int64_t n, const void *x, double *out);
+int qlg_scal(qlg_ctx ctx, int dtype, int64_t n, const double *alpha, void *x);
+int qlg_axpy(qlg_ctx ctx, int dtype, int64_t n, const double *alpha,
+             const void *x, void *y);
+/* y <- beta*y + sum_i coefs[i] * bases[i]; bases is a host array of k device
+ * pointers. */
+int qlg_linear_combine(qlg_ctx ctx, int dtype, int64_t n, int64_t k,
+                       const double *coefs, const void *const *bases,
+                       const double *beta, void *y);
+
+/* ---- Krylov solvers ---------------------------------------------------------
+ * qlg_lanczos_gs: LanczosSolver, lanczos_vmps_solver_impl.h:114-248 (plain
+ * three-term recurrence, no reorthogonalisation, stop when
+ * E_old - E_new < error, or the Krylov dimension reaches the vector size, or
+ * m == max_iterations - 1).  psi_inout holds the initial state on entry and
+ * the normalised ground state on return.
+ * qlg_lanczos_expmv: LanczosExpmvSolver,
+ * include/qlmps/algorithm/tdvp/lanczos_expmv_solver_impl.h:70-184
+ * (psi <- exp(-i*tau*H_eff) psi, norm of psi kept; QLG_C128 plans only).  */
+int qlg_lanczos_gs(qlg_plan plan, void *psi_inout,
+                   const qlg_lanczos_params *params, qlg_lanczos_result *res);
+int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau,
+                      const qlg_lanczos_params *params, int64_t *iters);
+
+/* ---- host helpers (FP64) ------------------------------------------------------
+ * qlg_tridiag_gs: TridiagGsSolver, lanczos_vmps_solver_impl.h:306-355
+ * (lowest eigenpair of the symmetric tridiagonal T(a[0..n), b[0..n-1));
+ * vec may be NULL for jobz 'N').
+ * qlg_tridiag_expme1: TridiagExpme1Solver, lanczos_expmv_solver_impl.h:199-240
+ * (exp(-i*tau*T) e_1, res = n interleaved complex values).               */
+int qlg_tridiag_gs(const double *a, const double *b, int64_t n, char jobz,
+                   double *eng, double *vec);
+int qlg_tridiag_expme1(const double *a, const double *b, int64_t n,
+                       double tau, double *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QLGPU_H */

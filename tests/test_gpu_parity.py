@@ -1,0 +1,212 @@
+"""GPU parity: the HIP matvec and Krylov solvers (through the C ABI) against the
+oracle on identical seeded inputs.
+
+Bars: block structure bit-exact (same blocks, same canonical order); FP64
+values within 1e-12 relative (max-abs error over max-abs value); Lanczos
+energies within 1e-12 relative of the oracle and 1e-8 of dense dsyev (the
+reference KAT bar); expmv within 1e-13 relative of the dense exponential.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.bsten import BSTensor, Contract, random_fill
+from oracle.dmrg import init_envs, random_mps, UpdateSiteLenvs
+from oracle.lanczos import (LanczosExpmvSolver as OExpmv, LanczosParams as OParams, LanczosSolver as OLanczos,
+                            eff_ham_mul_single_site_state, eff_ham_mul_two_site_state, two_site_matvec_flops)
+from oracle.models import build_mpo, heisenberg_chain, hubbard_cylinder
+from tests import refkat
+from tests.convert import to_device, to_oracle
+from tests.workloads import synthetic_eff_ham
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from ultradmrg_amd import tensor as T
+    from ultradmrg_amd.solver import EffectiveHamiltonian, LanczosParams
+
+
+def rel_err(a: BSTensor, b: BSTensor) -> float:
+    assert sorted(a.blocks) == sorted(b.blocks), "block structure differs"
+    num = max(float(np.abs(a.blocks[k] - b.blocks[k]).max()) for k in a.blocks)
+    den = max(float(np.abs(b.blocks[k]).max()) for k in b.blocks)
+    return num / den
+
+
+def gpu_matvec(eff, psi):
+    d_eff = [to_device(t) for t in eff]
+    d_psi = to_device(psi)
+    H = EffectiveHamiltonian(d_eff, d_psi)
+    out = H @ d_psi
+    return H, out
+
+
+@pytest.mark.parametrize("kind,D,nh,seed", [("heisenberg", 64, 4, 1), ("heisenberg", 300, 6, 2),
+                                             ("hubbard", 48, 2, 3), ("hubbard", 160, 3, 4)])
+def test_two_site_matvec_parity(kind, D, nh, seed):
+    print(f"[case] matvec {kind} D={D} seed={seed}", flush=True)
+    eff, psi = synthetic_eff_ham(kind, D, seed=seed, nsect_half=nh)
+    ref = eff_ham_mul_two_site_state(eff, psi)
+    H, out = gpu_matvec(eff, psi)
+    assert H.layout_coords == sorted(ref.blocks)
+    assert rel_err(to_oracle(out, ref), ref) < 1e-12
+    algo, execf = H.flops()
+    assert algo == pytest.approx(two_site_matvec_flops(eff, psi), rel=1e-12)
+
+
+def test_matvec_reference_kat_operators():
+    print("[case] matvec KAT operators D=20", flush=True)
+    L, Wl, Wr, R, (Din, dout, Dout) = refkat.kat_eff_ham(20, 2, 2)
+    psi = random_fill(BSTensor([Din, dout, dout, Dout]), (0,), np.random.default_rng(0))
+    ref = eff_ham_mul_two_site_state([L, Wl, Wr, R], psi)
+    _, out = gpu_matvec([L, Wl, Wr, R], psi)
+    assert rel_err(to_oracle(out, ref), ref) < 1e-12
+
+
+@pytest.mark.parametrize("which", ["heisenberg", "hubbard"])
+def test_one_site_matvec_parity(which):
+    print(f"[case] one-site matvec {which}", flush=True)
+    if which == "heisenberg":
+        m, n, total = heisenberg_chain(12), 12, (0,)
+    else:
+        m, n, total = hubbard_cylinder(3, 2), 6, (6, 0)
+    mpo = build_mpo(m)
+    mps = random_mps(m, total, 20, seed=77)
+    lenvs, renvs = init_envs(mps, mpo)
+    site = n // 2
+    for i in range(site):
+        lenvs[i + 1] = UpdateSiteLenvs(lenvs[i], mps[i], mpo[i])
+    eff = [lenvs[site], mpo[site], renvs[n - site - 1]]
+    psi = mps[site]
+    ref = eff_ham_mul_single_site_state(eff, psi)
+    d = [to_device(t) for t in eff]
+    H = EffectiveHamiltonian(d, to_device(psi))
+    out = H @ to_device(psi)
+    assert H.layout_coords == sorted(ref.blocks)
+    assert rel_err(to_oracle(out, ref), ref) < 1e-12
+
+
+def _dense_two_site(L, Wl, Wr, R):
+    t = np.einsum("xwa,wsty->xatsy", L.to_dense(), Wl.to_dense())
+    t = np.einsum("xatsy,yuvz->xtvzasu", t, Wr.to_dense())
+    H = np.einsum("xtvzasu,bzc->xtvbasuc", t, R.to_dense())
+    n = int(np.prod(H.shape[:4]))
+    return H.reshape(n, n)
+
+
+@pytest.mark.parametrize("params", [(1e-9, 200), (1e-16, 20)])
+def test_lanczos_reference_kat(params):
+    """test_lanczos_solver.cc:114-167 through the GPU solver: |E - dsyev| < 1e-8, both stop rules."""
+    print(f"[case] lanczos KAT {params}", flush=True)
+    L, Wl, Wr, R, (Din, dout, Dout) = refkat.kat_eff_ham(20, 2, 2)
+    psi = random_fill(BSTensor([Din, dout, dout, Dout]), (0,), np.random.default_rng(0))
+    d = [to_device(t) for t in (L, Wl, Wr, R)]
+    H = EffectiveHamiltonian(d, to_device(psi))
+    res = H.lanczos(to_device(psi), LanczosParams(*params))
+    w0 = np.linalg.eigvalsh(_dense_two_site(L, Wl, Wr, R))[0]
+    assert abs(res.gs_eng - w0) < 1e-8
+    ores = OLanczos([L, Wl, Wr, R], psi, eff_ham_mul_two_site_state, OParams(*params))
+    assert res.iters == ores.iters
+    assert abs(res.gs_eng - ores.gs_eng) <= 1e-12 * abs(ores.gs_eng)
+    if params[1] == 20:
+        assert res.iters == 20
+
+
+def _real_env_problem(model, n, D, seed, total):
+    mpo = build_mpo(model)
+    mps = random_mps(model, total, D, seed=seed)
+    lenvs, renvs = init_envs(mps, mpo)
+    # move the left environment to the middle
+    mid = n // 2 - 1
+    # right-canonical from 1; left-canonicalise 0..mid-1 by SVD is not needed for H_eff itself
+    for i in range(mid):
+        lenvs[i + 1] = UpdateSiteLenvs(lenvs[i], mps[i], mpo[i])
+    eff = [lenvs[mid], mpo[mid], mpo[mid + 1], renvs[n - mid - 2]]
+    psi = Contract(mps[mid], mps[mid + 1], ([2], [0]))
+    return eff, psi
+
+
+@pytest.mark.parametrize("which", ["heisenberg", "hubbard"])
+def test_lanczos_parity_real_envs(which):
+    print(f"[case] lanczos real envs {which}", flush=True)
+    if which == "heisenberg":
+        eff, psi = _real_env_problem(heisenberg_chain(14), 14, 24, 1001, (0,))
+    else:
+        eff, psi = _real_env_problem(hubbard_cylinder(3, 2), 6, 24, 1002, (6, 0))
+    prm = (1e-10, 100)
+    ores = OLanczos(eff, psi, eff_ham_mul_two_site_state, OParams(*prm))
+    d = [to_device(t) for t in eff]
+    H = EffectiveHamiltonian(d, to_device(psi))
+    res = H.lanczos(to_device(psi), LanczosParams(*prm))
+    assert abs(res.gs_eng - ores.gs_eng) <= 1e-12 * abs(ores.gs_eng)
+    assert abs(res.iters - ores.iters) <= 1
+    g = to_oracle(res.gs_vec, ores.gs_vec)
+    ov = sum(float(np.sum(g.blocks[k] * ores.gs_vec.blocks[k])) for k in g.blocks)
+    assert abs(abs(ov) - 1.0) < 1e-9
+
+
+def test_vector_kernels():
+    print("[case] vector kernels", flush=True)
+    rng = np.random.default_rng(9)
+    for cplx in (False, True):
+        n = 1_000_003
+        x = rng.normal(size=n) + (1j * rng.normal(size=n) if cplx else 0)
+        y = rng.normal(size=n) + (1j * rng.normal(size=n) if cplx else 0)
+        dx, dy = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+        assert abs(T.dot(dx, dy) - np.sum(x * np.conj(y))) < 1e-9 * n ** 0.5
+        assert abs(T.nrm2(dx) - np.linalg.norm(x)) < 1e-12 * np.linalg.norm(x)
+        al = 0.3 - (0.7j if cplx else 0)
+        T.axpy(al, dx, dy)
+        assert np.allclose(dy.cpu().numpy(), y + al * x, rtol=1e-14, atol=1e-14)
+        T.scal(2.5, dx)
+        assert np.allclose(dx.cpu().numpy(), 2.5 * x, rtol=1e-15, atol=0)
+        bases = [torch.from_numpy(rng.normal(size=n) + (1j * rng.normal(size=n) if cplx else 0)).cuda()
+                 for _ in range(5)]
+        coefs = list(rng.normal(size=5) + (1j * rng.normal(size=5) if cplx else 0))
+        out = torch.from_numpy(y.copy()).cuda()
+        T.linear_combine(coefs, bases, 0.0, out)
+        ref = sum(c * b.cpu().numpy() for c, b in zip(coefs, bases))
+        assert np.allclose(out.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
+        # determinism: the same reduction twice is bit-identical
+        assert T.dot(dx, dy) == T.dot(dx, dy)
+
+
+def test_complex_matvec_and_expmv_kat():
+    """test_lanczos_expmv_solver.cc:221-315 through the GPU: relative error < 1e-13."""
+    print("[case] complex expmv KAT", flush=True)
+    refkat.srand(0)
+    L, Wl, Wr, R, (Din, dout, Dout) = refkat.kat_eff_ham(10, 2, 2, complex_=True)
+    psi = random_fill(BSTensor([Din, dout, dout, Dout], np.complex128), (0,), np.random.default_rng(4), True)
+    ref = eff_ham_mul_two_site_state([L, Wl, Wr, R], psi)
+    d = [to_device(t) for t in (L, Wl, Wr, R)]
+    H = EffectiveHamiltonian(d, to_device(psi))
+    out = H @ to_device(psi)
+    assert rel_err(to_oracle(out, ref), ref) < 1e-12
+    v = psi.to_dense().reshape(-1)
+    Hd = _dense_two_site(L, Wl, Wr, R)
+    w, V = np.linalg.eigh(Hd)
+    exact = V @ (np.exp(-1j * 0.1 * w) * (V.conj().T @ v))
+    res = H.expmv(to_device(psi), 0.1, LanczosParams(1e-14, 100))
+    got = to_oracle(res.expmv, psi).to_dense().reshape(-1)
+    assert np.linalg.norm(got - exact) / np.linalg.norm(exact) < 1e-13
+    ores = OExpmv([L, Wl, Wr, R], psi, eff_ham_mul_two_site_state, 0.1, OParams(1e-14, 100))
+    assert res.iters == ores.iters
+
+
+def test_matvec_linearity_at_bench_size():
+    """Size-independent property at the benchmark's D=2000 Hubbard shape:
+    H(a x + b y) = a Hx + b Hy, and repeated matvecs are bit-identical."""
+    print("[case] linearity D=2000 hubbard", flush=True)
+    eff, psi = synthetic_eff_ham("hubbard", 2000, seed=1003)
+    d = [to_device(t) for t in eff]
+    x = to_device(psi)
+    H = EffectiveHamiltonian(d, x)
+    y = H.new_vector()
+    y.data.copy_(torch.randn(y.data.shape, dtype=torch.float64, device="cuda"))
+    z = H.new_vector()
+    z.data.copy_(0.7 * x.data - 1.3 * y.data)
+    hx, hy, hz = H @ x, H @ y, H @ z
+    hx2 = H @ x
+    assert torch.equal(hx.data, hx2.data)
+    diff = (hz.data - (0.7 * hx.data - 1.3 * hy.data)).abs().max().item()
+    assert diff <= 1e-12 * hz.data.abs().max().item()

@@ -17,8 +17,10 @@ def gaussian_bond(qns_site, width, D, nsect_half, qn_len=1):
         qs = [(2 * k,) for k in range(-nsect_half, nsect_half + 1)]
         w = np.array([np.exp(-(q[0] / 2) ** 2 / (2 * width ** 2)) for q in qs])
     else:
-        qs = [(n, 2 * s) for n in range(-nsect_half, nsect_half + 1) for s in range(-nsect_half, nsect_half + 1)]
-        w = np.array([np.exp(-(q[0] ** 2 + (q[1] / 2) ** 2) / (2 * width ** 2)) for q in qs])
+        # (N, 2Sz) with N + 2Sz even: one-electron moves change both by one unit
+        qs = [(n, m) for n in range(-nsect_half, nsect_half + 1) for m in range(-nsect_half, nsect_half + 1)
+              if (n + m) % 2 == 0]
+        w = np.array([np.exp(-(q[0] ** 2 + q[1] ** 2) / (2 * width ** 2)) for q in qs])
     degs = np.maximum(1, np.round(w / w.sum() * D)).astype(int)
     return [(q, int(d)) for q, d in zip(qs, degs)]
 

@@ -1,0 +1,285 @@
+// extern "C" entry points of libqlgpu (declared in include/qlgpu.h).  Every
+// call converts library exceptions into status codes and records the message
+// on the context.
+#include <cstring>
+#include <memory>
+
+#include "krylov.h"
+#include "plan.h"
+#include "tridiag.h"
+#include "vecops.h"
+
+using namespace qlg;
+
+struct qlg_ctx_s {
+  Ctx c;
+};
+struct qlg_plan_s {
+  std::unique_ptr<Plan> p;
+};
+
+namespace {
+thread_local std::string g_noctx_error;
+
+template <class F>
+int guarded(Ctx *c, F &&f) {
+  try {
+    if (c) QLG_HIP(hipSetDevice(c->device));
+    f();
+    return QLG_OK;
+  } catch (const Error &e) {
+    (c ? c->last_error : g_noctx_error) = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    (c ? c->last_error : g_noctx_error) = "host allocation failed";
+    return QLG_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    (c ? c->last_error : g_noctx_error) = e.what();
+    return QLG_ERR_INVALID;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int qlg_ctx_create(int device, void *stream, qlg_ctx *out) {
+  if (!out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  auto h = new (std::nothrow) qlg_ctx_s();
+  if (!h) return QLG_ERR_ALLOC;
+  int rc = guarded(nullptr, [&] {
+    int ndev = 0;
+    QLG_HIP(hipGetDeviceCount(&ndev));
+    QLG_CHECK(device >= 0 && device < ndev, "device index out of range");
+    h->c.device = device;
+    QLG_HIP(hipSetDevice(device));
+    if (stream) {
+      h->c.stream = static_cast<hipStream_t>(stream);
+    } else {
+      QLG_HIP(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
+      h->c.own_stream = true;
+    }
+    h->c.partials.ensure(2 * kMaxPartials * sizeof(double));
+    h->c.scalars.ensure(64 * sizeof(double));
+    QLG_HIP(hipHostMalloc((void **)&h->c.host_scalars, 64 * sizeof(double), hipHostMallocDefault));
+  });
+  if (rc != QLG_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return QLG_OK;
+}
+
+int qlg_ctx_destroy(qlg_ctx ctx) {
+  if (!ctx) return QLG_OK;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  if (ctx->c.host_scalars) (void)hipHostFree(ctx->c.host_scalars);
+  if (ctx->c.own_stream && ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return QLG_OK;
+}
+
+const char *qlg_last_error(qlg_ctx ctx) { return ctx ? ctx->c.last_error.c_str() : g_noctx_error.c_str(); }
+
+int qlg_ctx_synchronize(qlg_ctx ctx) {
+  if (!ctx) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] { QLG_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+static int make_plan(qlg_ctx ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
+                     const qlg_bsten *psi, int precision, qlg_plan *out) {
+  if (!ctx || !out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    QLG_CHECK(precision == QLG_PREC_F64 || precision == QLG_PREC_BF16X3 || precision == QLG_PREC_BF16X6,
+              "unknown precision");
+    auto h = std::make_unique<qlg_plan_s>();
+    h->p.reset(build_plan(&ctx->c, nphys, L, W, R, psi, precision));
+    *out = h.release();
+  });
+}
+
+int qlg_plan_two_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *Wl, const qlg_bsten *Wr, const qlg_bsten *R,
+                      const qlg_bsten *psi, int precision, qlg_plan *out) {
+  const qlg_bsten *W[2] = {Wl, Wr};
+  return make_plan(ctx, 2, L, W, R, psi, precision, out);
+}
+
+int qlg_plan_one_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *W, const qlg_bsten *R, const qlg_bsten *psi,
+                      int precision, qlg_plan *out) {
+  const qlg_bsten *Ws[2] = {W, nullptr};
+  return make_plan(ctx, 1, L, Ws, R, psi, precision, out);
+}
+
+int qlg_plan_destroy(qlg_plan plan) {
+  if (!plan) return QLG_OK;
+  (void)hipSetDevice(plan->p->ctx->device);
+  (void)hipStreamSynchronize(plan->p->ctx->stream);
+  delete plan;
+  return QLG_OK;
+}
+
+int qlg_plan_set_precision(qlg_plan plan, int precision) {
+  if (!plan) return QLG_ERR_INVALID;
+  return guarded(plan->p->ctx, [&] {
+    QLG_CHECK(precision == QLG_PREC_F64 || precision == QLG_PREC_BF16X3 || precision == QLG_PREC_BF16X6,
+              "unknown precision");
+    plan->p->precision = precision;
+  });
+}
+
+int qlg_plan_flops(qlg_plan plan, double *algo_flops, double *exec_flops) {
+  if (!plan) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  if (algo_flops) *algo_flops = p.algo_flops;
+  if (exec_flops) *exec_flops = p.st[0].flops + p.st[1].flops + p.st[2].flops;
+  return QLG_OK;
+}
+
+int qlg_plan_vector_layout(qlg_plan plan, int64_t *nblocks, int64_t *size, int32_t *coords, int64_t *offsets) {
+  if (!plan) return QLG_ERR_INVALID;
+  const Layout &v = plan->p->vec;
+  if (nblocks) *nblocks = (int64_t)v.blk.size();
+  if (size) *size = v.size;
+  for (size_t b = 0; b < v.blk.size(); ++b) {
+    if (coords)
+      for (int i = 0; i < v.rank; ++i) coords[b * v.rank + i] = v.blk[b][i];
+    if (offsets) offsets[b] = v.off[b];
+  }
+  return QLG_OK;
+}
+
+int qlg_plan_workspace(qlg_plan plan, int64_t *bytes) {
+  if (!plan || !bytes) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  int64_t b = (int64_t)(p.X.bytes + p.Y.bytes + p.C.bytes + p.basis.bytes);
+  for (auto &s : p.st) b += (int64_t)(s.d_tiles.bytes + s.d_tasks.bytes + s.d_segs.bytes);
+  *bytes = b;
+  return QLG_OK;
+}
+
+int qlg_plan_enable_timing(qlg_plan plan, int on) {
+  if (!plan) return QLG_ERR_INVALID;
+  plan->p->timing = on != 0;
+  return QLG_OK;
+}
+
+int qlg_plan_stage_ms(qlg_plan plan, double *ms3) {
+  if (!plan || !ms3) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.timing && p.ev[3], "timing is not enabled or no matvec has run");
+    QLG_HIP(hipEventSynchronize(p.ev[3]));
+    for (int i = 0; i < 3; ++i) {
+      float ms = 0;
+      QLG_HIP(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
+      ms3[i] = ms;
+    }
+  });
+}
+
+int qlg_plan_stage_flops(qlg_plan plan, double *flops3) {
+  if (!plan || !flops3) return QLG_ERR_INVALID;
+  for (int i = 0; i < 3; ++i) flops3[i] = plan->p->st[i].flops;
+  return QLG_OK;
+}
+
+int qlg_matvec(qlg_plan plan, const void *psi, void *out) {
+  if (!plan || !psi || !out) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
+    p.matvec(static_cast<const double *>(psi), static_cast<double *>(out));
+  });
+}
+
+int qlg_dot(qlg_ctx ctx, int dtype, int64_t n, const void *x, const void *y, double *out) {
+  if (!ctx || !out || n < 0) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    Ctx &c = ctx->c;
+    vec_dot(c, dtype, n, static_cast<const double *>(x), static_cast<const double *>(y), c.scalars.as<double>());
+    const int nv = dtype == QLG_C128 ? 2 : 1;
+    QLG_HIP(hipMemcpyAsync(c.host_scalars, c.scalars.p, nv * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < nv; ++i) out[i] = c.host_scalars[i];
+  });
+}
+
+int qlg_nrm2(qlg_ctx ctx, int dtype, int64_t n, const void *x, double *out) {
+  if (!ctx || !out || n < 0) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    Ctx &c = ctx->c;
+    vec_nrm2(c, dtype, n, static_cast<const double *>(x), c.scalars.as<double>());
+    QLG_HIP(hipMemcpyAsync(c.host_scalars, c.scalars.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+    out[0] = c.host_scalars[0];
+  });
+}
+
+int qlg_scal(qlg_ctx ctx, int dtype, int64_t n, const double *alpha, void *x) {
+  if (!ctx || !alpha || n < 0) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    vec_scal(ctx->c, dtype, n, alpha[0], dtype == QLG_C128 ? alpha[1] : 0.0, static_cast<double *>(x));
+  });
+}
+
+int qlg_axpy(qlg_ctx ctx, int dtype, int64_t n, const double *alpha, const void *x, void *y) {
+  if (!ctx || !alpha || n < 0) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    vec_axpy(ctx->c, dtype, n, alpha[0], dtype == QLG_C128 ? alpha[1] : 0.0, static_cast<const double *>(x),
+             static_cast<double *>(y));
+  });
+}
+
+int qlg_linear_combine(qlg_ctx ctx, int dtype, int64_t n, int64_t k, const double *coefs, const void *const *bases,
+                       const double *beta, void *y) {
+  if (!ctx || !coefs || !bases || !beta || n < 0 || k < 0) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    Ctx &c = ctx->c;
+    const bool cplx = dtype == QLG_C128;
+    DevBuf cb, pb;
+    cb.ensure(std::max<int64_t>(k, 1) * (cplx ? 2 : 1) * sizeof(double));
+    pb.ensure(std::max<int64_t>(k, 1) * sizeof(double *));
+    if (k > 0) {
+      QLG_HIP(hipMemcpyAsync(cb.p, coefs, k * (cplx ? 2 : 1) * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      QLG_HIP(hipMemcpyAsync(pb.p, bases, k * sizeof(void *), hipMemcpyHostToDevice, c.stream));
+    }
+    const double br = beta[0], bi = cplx ? beta[1] : 0.0;
+    vec_lincomb(c, dtype, n, (int)k, cb.as<double>(), pb.as<const double *>(), cplx, br, bi,
+                br != 0.0 || bi != 0.0, static_cast<double *>(y));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int qlg_lanczos_gs(qlg_plan plan, void *psi_inout, const qlg_lanczos_params *params, qlg_lanczos_result *res) {
+  if (!plan || !psi_inout || !params || !res) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
+    lanczos_gs(p, static_cast<double *>(psi_inout), *params, *res);
+  });
+}
+
+int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau, const qlg_lanczos_params *params, int64_t *iters) {
+  if (!plan || !psi_inout || !params || !iters) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
+    lanczos_expmv(p, static_cast<double *>(psi_inout), tau, *params, *iters);
+  });
+}
+
+int qlg_tridiag_gs(const double *a, const double *b, int64_t n, char jobz, double *eng, double *vec) {
+  if (!a || (n > 1 && !b) || !eng || n < 1 || (jobz != 'N' && jobz != 'V') || (jobz == 'V' && !vec))
+    return QLG_ERR_INVALID;
+  return tridiag_gs(n, a, b, jobz == 'V', eng, vec) ? QLG_OK : QLG_ERR_NUMERIC;
+}
+
+int qlg_tridiag_expme1(const double *a, const double *b, int64_t n, double tau, double *res) {
+  if (!a || (n > 1 && !b) || !res || n < 1) return QLG_ERR_INVALID;
+  return tridiag_expme1(n, a, b, tau, res) ? QLG_OK : QLG_ERR_NUMERIC;
+}
+
+}  // extern "C"

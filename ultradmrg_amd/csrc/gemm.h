@@ -1,0 +1,56 @@
+// Grouped, segmented GEMM task lists shared by the planner and the kernels.
+//
+// A task computes one output panel C[M x N] (row-major, ldc) as the sum over
+// its K-segments of A_s[M x K_s] * op(B_s), where A_s is row-major (lda) and
+// op(B_s) is B_s[K_s x N] (row-major, ldb) or B_s^T with B_s[N x K_s]
+// (row-major, ldb).  Segments are summed in list order inside one workgroup,
+// so results are deterministic and no atomics are used.  Output panels are
+// overwritten.
+//
+// Operand addresses are (base slot, element offset) pairs; the launcher
+// passes the slot pointers by value, so a task list can be reused for every
+// matvec whatever buffers the Krylov vectors live in.  Slot 0 is the null
+// pointer: an offset in slot 0 is an absolute element address.
+#pragma once
+#include <cstdint>
+
+namespace qlg {
+
+enum BaseSlot : uint8_t {
+  SLOT_ABS = 0,
+  SLOT_IN = 1,    // Krylov input vector
+  SLOT_OUT = 2,   // Krylov output vector
+  SLOT_L = 3,     // left environment
+  SLOT_R = 4,     // right environment
+  SLOT_X = 5,     // first intermediate
+  SLOT_Y = 6,     // second intermediate
+  SLOT_C = 7,     // combined MPO coefficient matrices
+  NSLOTS = 8
+};
+
+struct GemmSeg {
+  int64_t a_off, b_off;
+  int64_t lda, ldb;
+  int32_t K;
+  uint8_t a_slot, b_slot, transB, pad;
+};
+
+struct GemmTask {
+  int64_t c_off;
+  int64_t ldc;
+  int32_t M, N;
+  int32_t seg_begin, seg_count;
+  uint8_t c_slot, pad[7];
+};
+
+struct GemmTile {
+  int32_t task;
+  int32_t m0, n0;
+  int32_t pad;
+};
+
+struct Slots {
+  double *p[NSLOTS];
+};
+
+}  // namespace qlg

@@ -1,0 +1,224 @@
+// Krylov drivers on the GPU: Lanczos ground state and Lanczos exp(-i tau H)
+// psi.  The Krylov basis lives in one plan-owned device buffer; vector work
+// and the matvec run on the context stream; each iteration performs exactly
+// one device-to-host copy (the new diagonal element and the residual norm)
+// and solves the small FP64 tridiagonal problem on the host.
+//
+// Recurrences follow include/qlmps/algorithm/vmps/lanczos_vmps_solver_impl.h:114-248
+// and include/qlmps/algorithm/tdvp/lanczos_expmv_solver_impl.h:70-184:
+//   gamma_m = H v_{m-1} - a_{m-1} v_{m-1} - b_{m-2} v_{m-2},  b_{m-1} = ||gamma_m||,
+//   v_m = gamma_m / b_{m-1}.
+// The residual of iteration m+1 is formed right after a_m is known (reading
+// a_m and b_{m-1} from device memory), so its norm arrives with a_m in the
+// same copy.  If the solver stops at m, that speculative residual is simply
+// discarded: the ground state only combines v_0..v_m.
+#include <cmath>
+#include <vector>
+
+#include "krylov.h"
+#include "tridiag.h"
+#include "vecops.h"
+
+namespace qlg {
+
+namespace {
+
+struct Workspace {
+  Plan &p;
+  int64_t n, nd;
+  int64_t nslots;
+  double *slot0;
+  double *dev_a, *dev_b, *dev_tmp;  // device scalars
+  Workspace(Plan &pl, int64_t maxit) : p(pl) {
+    n = p.vec.size;
+    nd = n * elem_doubles(p.dtype);
+    nslots = maxit + 1;
+    if (p.basis_vectors < nslots) {
+      p.basis.release();
+      p.basis.ensure((size_t)nslots * nd * sizeof(double));
+      p.basis_vectors = nslots;
+    }
+    slot0 = p.basis.as<double>();
+    p.scal_buf.ensure((size_t)(2 * (maxit + 2) + 8) * sizeof(double));
+    dev_a = p.scal_buf.as<double>();
+    dev_b = dev_a + (maxit + 2);
+    dev_tmp = dev_b + (maxit + 2);
+    p.coef_buf.ensure((size_t)(2 * (maxit + 2)) * sizeof(double));
+    p.ptr_buf.ensure((size_t)(maxit + 2) * sizeof(double *));
+  }
+  double *slot(int64_t i) { return slot0 + i * nd; }
+  // copy nvals doubles from device to host and wait
+  void fetch(const double *dev, double *host, int nvals) {
+    Ctx &c = *p.ctx;
+    QLG_HIP(hipMemcpyAsync(c.host_scalars, dev, nvals * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < nvals; ++i) host[i] = c.host_scalars[i];
+  }
+  // y <- sum_i coef_i * slot(i) for i < k (real or complex coefficients)
+  void combine(int64_t k, const double *coefs, bool cplx_coef, double *y) {
+    Ctx &c = *p.ctx;
+    std::vector<const double *> ptrs(k);
+    for (int64_t i = 0; i < k; ++i) ptrs[i] = slot(i);
+    QLG_HIP(hipMemcpyAsync(p.coef_buf.p, coefs, k * (cplx_coef ? 2 : 1) * sizeof(double), hipMemcpyHostToDevice,
+                           c.stream));
+    QLG_HIP(hipMemcpyAsync(p.ptr_buf.p, ptrs.data(), k * sizeof(double *), hipMemcpyHostToDevice, c.stream));
+    vec_lincomb(c, p.dtype, n, (int)k, p.coef_buf.as<double>(), p.ptr_buf.as<const double *>(), cplx_coef, 0.0, 0.0,
+                false, y);
+    // the host arrays above must stay alive until the copies are done
+    QLG_HIP(hipStreamSynchronize(c.stream));
+  }
+};
+
+}  // namespace
+
+void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos_result &res) {
+  QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
+  Ctx &c = *p.ctx;
+  const int64_t maxit = prm.max_iterations;
+  Workspace w(p, maxit);
+  const int64_t dim = w.n;
+  std::vector<double> a(maxit + 1, 0.0), b(maxit + 1, 0.0), N(maxit + 1, 0.0);
+  double h[2];
+
+  // v0 = psi / ||psi||   (QuasiNormalize, lanczos_vmps_solver_impl.h:141)
+  QLG_HIP(hipMemcpyAsync(w.slot(0), psi, w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  vec_nrm2(c, p.dtype, w.n, w.slot(0), w.dev_tmp);
+  w.fetch(w.dev_tmp, h, 1);
+  QLG_CHECK(h[0] > 0.0, "initial state has zero norm");
+  vec_scal(c, p.dtype, w.n, 1.0 / h[0], 0.0, w.slot(0));
+  // a_0 and the first residual
+  p.matvec(w.slot(0), w.slot(1));
+  vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0);
+  vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0, nullptr, nullptr, w.dev_b + 0);
+  QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  w.fetch(w.dev_tmp, h, 2);
+  a[0] = h[0];
+  double norm_gamma = h[1];
+  double energy0 = a[0];
+  int64_t m = 0;
+  while (true) {
+    m += 1;
+    // slot m holds gamma_m (un-normalised residual), norm_gamma = ||gamma_m||
+    if (norm_gamma == 0.0) {
+      if (m == 1) {
+        QLG_HIP(hipMemcpyAsync(psi, w.slot(0), w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        QLG_HIP(hipStreamSynchronize(c.stream));
+        res.iters = m;
+        res.gs_eng = energy0;
+        return;
+      }
+      std::vector<double> vec(m);
+      double e;
+      if (!tridiag_gs(m, a.data(), b.data(), true, &e, vec.data()))
+        throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+      w.combine(m, vec.data(), false, psi);
+      res.iters = m;
+      res.gs_eng = energy0;
+      return;
+    }
+    N[m] = norm_gamma * norm_gamma;
+    b[m - 1] = norm_gamma;
+    vec_scal(c, p.dtype, w.n, 1.0 / norm_gamma, 0.0, w.slot(m));
+    p.matvec(w.slot(m), w.slot(m + 1));
+    vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m);
+    // speculative residual for iteration m+1: H v_m - a_m v_m - b_{m-1} v_{m-1}
+    vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m, w.slot(m - 1), w.dev_b + (m - 1),
+                         w.dev_b + m);
+    QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    w.fetch(w.dev_tmp, h, 2);
+    a[m] = h[0];
+    norm_gamma = h[1];
+    double e_new;
+    if (!tridiag_gs(m + 1, a.data(), b.data(), false, &e_new, nullptr))
+      throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+    if ((energy0 - e_new) < prm.error || m == dim || m == maxit - 1) {
+      std::vector<double> vec(m + 1);
+      if (!tridiag_gs(m + 1, a.data(), b.data(), true, &e_new, vec.data()))
+        throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+      w.combine(m + 1, vec.data(), false, psi);
+      res.iters = m + 1;
+      res.gs_eng = e_new;
+      return;
+    }
+    energy0 = e_new;
+  }
+}
+
+void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &prm, int64_t &iters) {
+  QLG_CHECK(p.dtype == QLG_C128, "Lanczos expmv needs a complex (QLG_C128) plan");
+  QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
+  Ctx &c = *p.ctx;
+  const int64_t maxit = prm.max_iterations;
+  Workspace w(p, maxit);
+  const int64_t dim = w.n;
+  std::vector<double> a(maxit + 1, 0.0), b(maxit + 1, 0.0);
+  std::vector<double> coef(2 * (maxit + 1), 0.0), last(2 * (maxit + 1), 0.0);
+  double h[2];
+
+  // Normalize keeps the initial norm (lanczos_expmv_solver_impl.h:93)
+  QLG_HIP(hipMemcpyAsync(w.slot(0), psi, w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  vec_nrm2(c, p.dtype, w.n, w.slot(0), w.dev_tmp);
+  w.fetch(w.dev_tmp, h, 1);
+  const double initial_norm = h[0];
+  QLG_CHECK(initial_norm > 0.0, "initial state has zero norm");
+  vec_scal(c, p.dtype, w.n, 1.0 / initial_norm, 0.0, w.slot(0));
+  p.matvec(w.slot(0), w.slot(1));
+  // a_m = Re <H v_m | v_m>; the complex dot writes (re, im)
+  vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_tmp + 2);
+  QLG_HIP(hipMemcpyAsync(w.dev_a + 0, w.dev_tmp + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0, nullptr, nullptr, w.dev_b + 0);
+  QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  w.fetch(w.dev_tmp, h, 2);
+  a[0] = h[0];
+  double norm_gamma = h[1];
+  int64_t m = 0;
+  while (true) {
+    m += 1;
+    if (norm_gamma == 0.0) {
+      iters = m;
+      if (m == 1) {
+        // psi <- ||psi0|| exp(-i tau a_0) v_0
+        const double ph = -tau * a[0];
+        QLG_HIP(hipMemcpyAsync(psi, w.slot(0), w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        vec_scal(c, p.dtype, w.n, initial_norm * std::cos(ph), initial_norm * std::sin(ph), psi);
+        QLG_HIP(hipStreamSynchronize(c.stream));
+        return;
+      }
+      if (!tridiag_expme1(m, a.data(), b.data(), tau, coef.data()))
+        throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+      for (int64_t i = 0; i < 2 * m; ++i) coef[i] *= initial_norm;
+      w.combine(m, coef.data(), true, psi);
+      return;
+    }
+    b[m - 1] = norm_gamma;
+    vec_scal(c, p.dtype, w.n, 1.0 / norm_gamma, 0.0, w.slot(m));
+    p.matvec(w.slot(m), w.slot(m + 1));
+    vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_tmp + 2);
+    QLG_HIP(hipMemcpyAsync(w.dev_a + m, w.dev_tmp + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m, w.slot(m - 1), w.dev_b + (m - 1),
+                         w.dev_b + m);
+    QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    w.fetch(w.dev_tmp, h, 2);
+    a[m] = h[0];
+    norm_gamma = h[1];
+    std::fill(coef.begin(), coef.end(), 0.0);
+    if (!tridiag_expme1(m + 1, a.data(), b.data(), tau, coef.data()))
+      throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+    double dist = 0.0;
+    for (int64_t i = 0; i < 2 * (m + 1); ++i) dist += (coef[i] - last[i]) * (coef[i] - last[i]);
+    dist = std::sqrt(dist);
+    if (dist < prm.error || m == dim || m == maxit - 1) {
+      iters = m + 1;
+      for (int64_t i = 0; i < 2 * (m + 1); ++i) coef[i] *= initial_norm;
+      w.combine(m + 1, coef.data(), true, psi);
+      return;
+    }
+    last = coef;
+  }
+}
+
+}  // namespace qlg

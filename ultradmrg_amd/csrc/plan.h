@@ -1,0 +1,74 @@
+// Effective-Hamiltonian plan: sector pairing done once per site, three
+// grouped-GEMM stages replayed for every matvec.
+//
+// Stage 1  (L . psi):   for every L block (a', w, a) and state block
+//          (a, s1, s2, b) and every MPO / physical component, one GEMM
+//          X[a', b] = L[a', w, :] . psi[:, s1, s2, b]   (M=D_a', K=D_a, N=D_b),
+//          written as a contiguous chunk of the channel (a'-sector, b-sector).
+// Stage 2  (MPO):       per channel, Y = C . X with C the combined two-site MPO
+//          operator sum_y Wl[w,s1,t1,y] Wr[y,s2,t2,z] restricted to the
+//          channel's input chunks (w,s1,s2) and output rows (t1,t2,z).
+// Stage 3  (. R^T):     per output block (a', t1, t2, b'), the sum over
+//          (z, b) segments of Y[a', b] . R[b', z, b]^T.
+// The one-site form uses the same three stages with C = W.
+#pragma once
+#include <vector>
+
+#include "common.h"
+#include "gemm.h"
+
+namespace qlg {
+
+struct Stage {
+  std::vector<GemmTile> tiles;
+  std::vector<GemmTask> tasks;
+  std::vector<GemmSeg> segs;
+  DevBuf d_tiles, d_tasks, d_segs;
+  int bm = 64, bn = 64;
+  double flops = 0;      // useful 2*M*N*K issued by this stage
+  void upload(hipStream_t st);
+  void make_tiles(int bm_, int bn_);
+};
+
+struct Layout {
+  int rank = 0;
+  std::vector<Axis> ax;
+  std::vector<Coords> blk;
+  std::vector<int64_t> off;
+  std::unordered_map<Coords, int64_t, CoordsHash> find;
+  int64_t size = 0;
+  int64_t blk_elems(int64_t b) const {
+    int64_t n = 1;
+    for (int i = 0; i < rank; ++i) n *= ax[i].deg[blk[b][i]];
+    return n;
+  }
+};
+
+struct Plan {
+  Ctx *ctx = nullptr;
+  int nphys = 2;
+  int dtype = QLG_F64;
+  int precision = QLG_PREC_F64;
+  Ten L, W[2], R;
+  Layout vec;
+  Stage st[3];
+  DevBuf X, Y, C;
+  int64_t x_elems = 0, y_elems = 0, c_elems = 0;
+  double algo_flops = 0;
+  // Krylov workspace
+  DevBuf basis, coef_buf, ptr_buf, scal_buf;
+  int64_t basis_vectors = 0;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~Plan();
+  void matvec(const double *in, double *out);
+};
+
+Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
+                 const qlg_bsten *psi, int precision);
+
+void launch_grouped_gemm_f64(int bm, int bn, bool cplx, const GemmTile *tiles, const GemmTask *tasks,
+                             const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st);
+
+}  // namespace qlg

@@ -1,0 +1,227 @@
+"""Python mirror of the reference's effective-Hamiltonian / Krylov call surface,
+running on the HIP library.
+
+Reference names kept (``include/qlmps/algorithm/vmps/lanczos_vmps_solver_impl.h``,
+``include/qlmps/algorithm/tdvp/lanczos_expmv_solver_impl.h``,
+``include/qlmps/algorithm/lanczos_params.h``):
+
+* ``LanczosParams(error, max_iterations)`` (defaults 1e-7 / 200)
+* ``LanczosRes(iters, gs_eng, gs_vec)`` and ``ExpmvRes(iters, expmv)``
+* ``LanczosSolver(eff_ham, init_state, matvec, params)`` where ``eff_ham`` is
+  ``[L, W_l, W_r, R]`` (or ``[L, W, R]``) and ``matvec`` is
+  ``eff_ham_mul_two_site_state`` / ``eff_ham_mul_single_site_state``
+* ``LanczosExpmvSolver(eff_ham, init_state, matvec, step_length, params)``
+* ``TridiagGsSolver`` / ``TridiagExpme1Solver``
+
+Differences from the reference that a caller sees: the initial state is not
+consumed (it is left untouched), and the returned vectors are new device
+tensors in the plan's Krylov layout.  ``EffectiveHamiltonian`` holds the plan
+(the sector pairing) so that repeated solves with the same operators skip
+re-planning.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .tensor import BlockTensor, Context, default_context
+
+PRECISIONS = {"fp64": L.QLG_PREC_F64, "bf16x3": L.QLG_PREC_BF16X3, "bf16x6": L.QLG_PREC_BF16X6}
+
+
+@dataclass
+class LanczosParams:
+    error: float = 1.0e-7
+    max_iterations: int = 200
+
+
+@dataclass
+class LanczosRes:
+    iters: int
+    gs_eng: float
+    gs_vec: BlockTensor
+
+
+@dataclass
+class ExpmvRes:
+    iters: int
+    expmv: BlockTensor
+
+
+class EffectiveHamiltonian:
+    """A planned H_eff = L . W... . R acting on states with ``state``'s structure."""
+
+    def __init__(self, eff_ham: Sequence[BlockTensor], state: BlockTensor, precision: str = "fp64",
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context(state.data.device.index or 0)
+        self.ops = list(eff_ham)          # keep operand buffers alive
+        lib = self.ctx.lib
+        h = C.c_void_p()
+        descs = [t.descriptor() for t in self.ops]
+        sd = state.descriptor()
+        torch.cuda.synchronize()
+        if len(self.ops) == 4:
+            rc = lib.qlg_plan_two_site(self.ctx.h, *[C.byref(d) for d in descs], C.byref(sd),
+                                       PRECISIONS[precision], C.byref(h))
+        elif len(self.ops) == 3:
+            rc = lib.qlg_plan_one_site(self.ctx.h, *[C.byref(d) for d in descs], C.byref(sd),
+                                       PRECISIONS[precision], C.byref(h))
+        else:
+            raise ValueError("eff_ham must be [L, W_l, W_r, R] or [L, W, R]")
+        self.ctx.check(rc)
+        self.h = h
+        self.lib = lib
+        nb, size = C.c_int64(), C.c_int64()
+        self.ctx.check(lib.qlg_plan_vector_layout(h, C.byref(nb), C.byref(size), None, None))
+        coords = (C.c_int32 * max(nb.value * state.rank, 1))()
+        offs = (C.c_int64 * max(nb.value, 1))()
+        self.ctx.check(lib.qlg_plan_vector_layout(h, None, None, coords, offs))
+        r = state.rank
+        self.layout_coords = [tuple(coords[b * r:(b + 1) * r]) for b in range(nb.value)]
+        self.layout_offsets = list(offs[:nb.value])
+        self.size = size.value
+        self.axes = state.axes
+        self.dtype = state.data.dtype
+        self.device = state.data.device
+
+    # ----- layout helpers ------------------------------------------------------------
+    def new_vector(self, zero=True) -> BlockTensor:
+        buf = (torch.zeros if zero else torch.empty)(max(self.size, 1), dtype=self.dtype, device=self.device)
+        return BlockTensor(self.axes, self.layout_coords, buf, self.layout_offsets)
+
+    def adopt(self, state: BlockTensor) -> BlockTensor:
+        """Return ``state`` in the plan's Krylov layout (copying blocks if the layouts differ)."""
+        if state.coords == self.layout_coords and state.offsets == self.layout_offsets:
+            return state
+        out = self.new_vector()
+        pos = {c: o for c, o in zip(self.layout_coords, self.layout_offsets)}
+        for c, o in zip(state.coords, state.offsets):
+            n = state.block_size(c)
+            out.data[pos[c]:pos[c] + n] = state.data[o:o + n]
+        return out
+
+    def flops(self):
+        a, e = C.c_double(), C.c_double()
+        self.ctx.check(self.lib.qlg_plan_flops(self.h, C.byref(a), C.byref(e)))
+        return a.value, e.value
+
+    def workspace_bytes(self) -> int:
+        b = C.c_int64()
+        self.ctx.check(self.lib.qlg_plan_workspace(self.h, C.byref(b)))
+        return b.value
+
+    def set_precision(self, precision: str):
+        self.ctx.check(self.lib.qlg_plan_set_precision(self.h, PRECISIONS[precision]))
+
+    def enable_timing(self, on=True):
+        self.ctx.check(self.lib.qlg_plan_enable_timing(self.h, int(on)))
+
+    def stage_flops(self):
+        out = (C.c_double * 3)()
+        self.ctx.check(self.lib.qlg_plan_stage_flops(self.h, out))
+        return list(out)
+
+    def stage_ms(self):
+        out = (C.c_double * 3)()
+        self.ctx.check(self.lib.qlg_plan_stage_ms(self.h, out))
+        return list(out)
+
+    # ----- operations -------------------------------------------------------------------
+    def matvec_into(self, x: torch.Tensor, out: torch.Tensor, sync=True):
+        self.ctx.check(self.lib.qlg_matvec(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr())))
+        if sync:
+            self.ctx.synchronize()
+
+    def __matmul__(self, state: BlockTensor) -> BlockTensor:
+        x = self.adopt(state)
+        out = self.new_vector(zero=False)
+        torch.cuda.synchronize()
+        self.matvec_into(x.data, out.data)
+        return out
+
+    def lanczos(self, init_state: BlockTensor, params: LanczosParams) -> LanczosRes:
+        v = self.adopt(init_state)
+        work = self.new_vector(zero=False)
+        work.data.copy_(v.data)
+        torch.cuda.synchronize()
+        res = L.LanczosResult()
+        prm = L.LanczosParams(params.error, params.max_iterations)
+        self.ctx.check(self.lib.qlg_lanczos_gs(self.h, C.c_void_p(work.data.data_ptr()), C.byref(prm), C.byref(res)))
+        return LanczosRes(int(res.iters), float(res.gs_eng), work)
+
+    def expmv(self, init_state: BlockTensor, step_length: float, params: LanczosParams) -> ExpmvRes:
+        v = self.adopt(init_state)
+        work = self.new_vector(zero=False)
+        work.data.copy_(v.data)
+        torch.cuda.synchronize()
+        it = C.c_int64()
+        prm = L.LanczosParams(params.error, params.max_iterations)
+        self.ctx.check(self.lib.qlg_lanczos_expmv(self.h, C.c_void_p(work.data.data_ptr()), float(step_length),
+                                                  C.byref(prm), C.byref(it)))
+        return ExpmvRes(int(it.value), work)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qlg_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# reference-named entry points
+# ---------------------------------------------------------------------------
+def eff_ham_mul_two_site_state(eff_ham: Sequence[BlockTensor], state: BlockTensor) -> BlockTensor:
+    """``lanczos_vmps_solver_impl.h:260-289`` (plans, applies once, returns a new tensor)."""
+    return EffectiveHamiltonian(eff_ham, state) @ state
+
+
+def eff_ham_mul_single_site_state(eff_ham: Sequence[BlockTensor], state: BlockTensor) -> BlockTensor:
+    """``lanczos_vmps_solver_impl.h:291-304``."""
+    return EffectiveHamiltonian(eff_ham, state) @ state
+
+
+def LanczosSolver(eff_ham, init_state: BlockTensor, matvec=None, params: LanczosParams = LanczosParams(),
+                  precision: str = "fp64") -> LanczosRes:
+    """``lanczos_vmps_solver_impl.h:114-248``; ``eff_ham`` may be an EffectiveHamiltonian."""
+    H = eff_ham if isinstance(eff_ham, EffectiveHamiltonian) else EffectiveHamiltonian(eff_ham, init_state,
+                                                                                       precision)
+    return H.lanczos(init_state, params)
+
+
+def LanczosExpmvSolver(eff_ham, init_state: BlockTensor, matvec=None, step_length: float = 0.0,
+                       params: LanczosParams = LanczosParams()) -> ExpmvRes:
+    """``lanczos_expmv_solver_impl.h:70-184``."""
+    H = eff_ham if isinstance(eff_ham, EffectiveHamiltonian) else EffectiveHamiltonian(eff_ham, init_state)
+    return H.expmv(init_state, step_length, params)
+
+
+def TridiagGsSolver(a: Sequence[float], b: Sequence[float], n: int, jobz: str = "N"):
+    """``lanczos_vmps_solver_impl.h:306-355`` through the library's FP64 QL solver."""
+    lib = L.lib()
+    A = (C.c_double * n)(*a[:n])
+    B = (C.c_double * max(n - 1, 1))(*(list(b[:n - 1]) or [0.0]))
+    e = C.c_double()
+    v = (C.c_double * n)()
+    L.check(lib.qlg_tridiag_gs(A, B, n, jobz.encode(), C.byref(e), v if jobz == "V" else None))
+    return e.value, (np.array(v[:]) if jobz == "V" else None)
+
+
+def TridiagExpme1Solver(a: Sequence[float], b: Sequence[float], n: int, step_length: float) -> np.ndarray:
+    """``lanczos_expmv_solver_impl.h:199-240``."""
+    lib = L.lib()
+    A = (C.c_double * n)(*a[:n])
+    B = (C.c_double * max(n - 1, 1))(*(list(b[:n - 1]) or [0.0]))
+    out = (C.c_double * (2 * n))()
+    L.check(lib.qlg_tridiag_expme1(A, B, n, float(step_length), out))
+    z = np.array(out[:])
+    return z[0::2] + 1j * z[1::2]
