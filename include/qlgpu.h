@@ -85,6 +85,15 @@ int qlg_ctx_destroy(qlg_ctx ctx);
 const char *qlg_last_error(qlg_ctx ctx);
 int qlg_ctx_synchronize(qlg_ctx ctx);
 
+/* ---- device memory plumbing (for hosts without their own HIP allocator) --
+ * Allocations are plain hipMalloc memory on the context's device; copies run
+ * on the context stream and return after completion.                      */
+int qlg_device_alloc(qlg_ctx ctx, int64_t bytes, void **ptr);
+int qlg_device_free(qlg_ctx ctx, void *ptr);
+int qlg_memcpy_h2d(qlg_ctx ctx, void *dst, const void *src, int64_t bytes);
+int qlg_memcpy_d2h(qlg_ctx ctx, void *dst, const void *src, int64_t bytes);
+int qlg_memcpy_d2d(qlg_ctx ctx, void *dst, const void *src, int64_t bytes);
+
 /* ---- effective-Hamiltonian plans ----------------------------------------
  * Replaces the {L, W_l, W_r, R} operand vector and the function pointer
  * eff_ham_mul_two_site_state of

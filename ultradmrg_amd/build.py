@@ -74,9 +74,31 @@ def build(jobs: int = 8, debug: bool = False, verbose: bool = True) -> str:
     return LIB
 
 
+def build_cpp_tests(verbose: bool = True) -> list:
+    """Compile the C++ programs under tests/cpp (header-only layer users) with g++."""
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    out = []
+    if not os.path.isdir(tdir):
+        return out
+    for f in sorted(os.listdir(tdir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(tdir, f[:-4])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(ROOT, 'include')}", os.path.join(tdir, f),
+               "-o", exe, f"-L{HERE}", "-lqlgpu", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../ultradmrg_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+    if verbose:
+        print(f"built {len(out)} C++ test programs", file=sys.stderr)
+    return out
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("--debug", action="store_true")
     a = ap.parse_args()
     build(a.jobs, a.debug)
+    build_cpp_tests()

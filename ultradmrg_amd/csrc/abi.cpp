@@ -88,6 +88,42 @@ int qlg_ctx_synchronize(qlg_ctx ctx) {
   return guarded(&ctx->c, [&] { QLG_HIP(hipStreamSynchronize(ctx->c.stream)); });
 }
 
+int qlg_device_alloc(qlg_ctx ctx, int64_t bytes, void **ptr) {
+  if (!ctx || !ptr || bytes < 0) return QLG_ERR_INVALID;
+  *ptr = nullptr;
+  return guarded(&ctx->c, [&] {
+    if (hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1)) != hipSuccess) {
+      *ptr = nullptr;
+      throw Error(QLG_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+  });
+}
+
+int qlg_device_free(qlg_ctx ctx, void *ptr) {
+  if (!ctx) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    if (ptr) QLG_HIP(hipFree(ptr));
+  });
+}
+
+static int copy_sync(qlg_ctx ctx, void *dst, const void *src, int64_t bytes, hipMemcpyKind kind) {
+  if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    if (bytes > 0) QLG_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, kind, ctx->c.stream));
+    QLG_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int qlg_memcpy_h2d(qlg_ctx ctx, void *dst, const void *src, int64_t bytes) {
+  return copy_sync(ctx, dst, src, bytes, hipMemcpyHostToDevice);
+}
+int qlg_memcpy_d2h(qlg_ctx ctx, void *dst, const void *src, int64_t bytes) {
+  return copy_sync(ctx, dst, src, bytes, hipMemcpyDeviceToHost);
+}
+int qlg_memcpy_d2d(qlg_ctx ctx, void *dst, const void *src, int64_t bytes) {
+  return copy_sync(ctx, dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
 static int make_plan(qlg_ctx ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
                      const qlg_bsten *psi, int precision, qlg_plan *out) {
   if (!ctx || !out) return QLG_ERR_INVALID;

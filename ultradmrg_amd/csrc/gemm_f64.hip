@@ -42,6 +42,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
   const int tile_id = blockIdx.x;
   if (tile_id >= ntiles) return;
   const GemmTile tl = tiles[tile_id];
+  if (tl.task < 0) return;  // padding slot of the XCD-balanced launch order
   const GemmTask tk = tasks[tl.task];
   const int M = tk.M, N = tk.N;
 

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <set>
@@ -67,23 +69,41 @@ static void check_pair(const Axis &a, const Axis &b, const char *what) {
 }
 
 // ---------------------------------------------------------------------------
+// Launch order of the tiles.  Workgroup b of a launch runs on XCD b % 8, and
+// each XCD has its own L2: all tiles of one task (which share A row panels
+// and B column panels) are placed on one XCD, tasks are spread over the XCDs
+// by longest-processing-time-first, and each XCD walks its tasks from the
+// most to the least expensive.  Slots past an XCD's share are empty tiles.
 void Stage::make_tiles(int bm_, int bn_) {
+  constexpr int kXcd = 8;
   bm = bm_;
   bn = bn_;
   tiles.clear();
-  // tasks with the largest K first: long tiles start early, short ones fill the tail
   std::vector<int> order(tasks.size());
-  std::vector<int64_t> ktot(tasks.size(), 0);
+  std::vector<double> cost(tasks.size(), 0.0);
   for (size_t t = 0; t < tasks.size(); ++t) {
     order[t] = (int)t;
-    for (int s = 0; s < tasks[t].seg_count; ++s) ktot[t] += segs[tasks[t].seg_begin + s].K;
+    int64_t kp = 0;
+    for (int s = 0; s < tasks[t].seg_count; ++s) kp += (segs[tasks[t].seg_begin + s].K + 15) / 16 * 16;
+    const double nt = double((tasks[t].M + bm - 1) / bm) * double((tasks[t].N + bn - 1) / bn);
+    cost[t] = nt * (double(bm) * bn * std::max<int64_t>(kp, 16) + 4096.0);
   }
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ktot[a] > ktot[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<std::vector<GemmTile>> per(kXcd);
+  std::vector<double> load(kXcd, 0.0);
   for (int t : order) {
+    const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += cost[t];
     const GemmTask &k = tasks[t];
     for (int m0 = 0; m0 < k.M; m0 += bm)
-      for (int n0 = 0; n0 < k.N; n0 += bn) tiles.push_back(GemmTile{t, m0, n0, 0});
+      for (int n0 = 0; n0 < k.N; n0 += bn) per[x].push_back(GemmTile{t, m0, n0, 0});
   }
+  size_t slots = 0;
+  for (auto &v : per) slots = std::max(slots, v.size());
+  tiles.reserve(slots * kXcd);
+  for (size_t s = 0; s < slots; ++s)
+    for (int x = 0; x < kXcd; ++x) tiles.push_back(s < per[x].size() ? per[x][s] : GemmTile{-1, 0, 0, 0});
+  while (!tiles.empty() && tiles.back().task < 0) tiles.pop_back();
 }
 
 void Stage::upload(hipStream_t st) {
@@ -138,20 +158,24 @@ static void autotune_tiles(Stage &s, bool cplx) {
   std::vector<int64_t> kpad(s.tasks.size(), 0);
   for (size_t t = 0; t < s.tasks.size(); ++t)
     for (int q = 0; q < s.tasks[t].seg_count; ++q) kpad[t] += (s.segs[s.tasks[t].seg_begin + q].K + 15) / 16 * 16;
+  const double ebytes = cplx ? 16.0 : 8.0, fmul = cplx ? 4.0 : 1.0;
   for (int si = 0; si < nshapes; ++si) {
     const int *sh = shapes[si];
-    double total = 0, longest = 0, ntiles = 0;
+    double flops = 0, bytes = 0, longest = 0, ntiles = 0;
     for (size_t t = 0; t < s.tasks.size(); ++t) {
       const GemmTask &k = s.tasks[t];
       const double nt = double((k.M + sh[0] - 1) / sh[0]) * double((k.N + sh[1] - 1) / sh[1]);
-      // one 16x16x4 FP64 MFMA per wave = 64 cycles; 4 waves per tile
-      const double tcost = double(sh[0]) * sh[1] * (double)std::max<int64_t>(kpad[t], 16) / (16.0 * 16 * 4 * 4) * 64.0 +
-                           1500.0;
-      total += nt * tcost;
-      longest = std::max(longest, tcost);
+      const double kp = (double)std::max<int64_t>(kpad[t], 16);
+      flops += nt * 2.0 * sh[0] * sh[1] * kp * fmul;
+      // operand panels streamed per tile + the output tile
+      bytes += nt * ((sh[0] + sh[1]) * kp + double(sh[0]) * sh[1]) * ebytes;
+      // one workgroup alone: 4 waves, one 64-cycle FP64 MFMA per 16x16x4 block each, 2.4 GHz
+      longest = std::max(longest, sh[0] * sh[1] * kp * fmul / 64.0 / 2.4e9);
       ntiles += nt;
     }
-    const double est = std::max(total / (256.0 * 2.0), longest);
+    // MFMA time at the FP64 peak, operand streaming at an L2-assisted 12 TB/s,
+    // and ~1.5 us of exposed latency per tile spread over 1024 resident tiles
+    const double est = std::max({flops / 78.6e12, bytes / 12e12, longest}) + ntiles * 1.5e-6 / 1024.0;
     if (est < best * 0.97) {
       best = est;
       bm = sh[0];
@@ -645,9 +669,36 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     slot_elems[SLOT_C] = p.c_elems;
     for (int i = 0; i < 3; ++i) validate_stage(p.st[i], slot_elems);
   }
+  // QLG_TILES="BMxBN,BMxBN,BMxBN" pins the tile shape of each stage (tuning runs)
+  const char *pin = std::getenv("QLG_TILES");
+  int pinned[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  if (pin) std::sscanf(pin, "%dx%d,%dx%d,%dx%d", &pinned[0][0], &pinned[0][1], &pinned[1][0], &pinned[1][1],
+                       &pinned[2][0], &pinned[2][1]);
   for (int i = 0; i < 3; ++i) {
-    autotune_tiles(p.st[i], p.dtype == QLG_C128);
+    if (pinned[i][0] > 0)
+      p.st[i].make_tiles(pinned[i][0], pinned[i][1]);
+    else
+      autotune_tiles(p.st[i], p.dtype == QLG_C128);
     p.st[i].upload(ctx->stream);
+  }
+  if (const char *v = std::getenv("QLG_VERBOSE"); v && v[0] == '1') {
+    for (int i = 0; i < 3; ++i) {
+      const Stage &s = p.st[i];
+      double padded = 0;
+      for (const GemmTile &t : s.tiles) {
+        if (t.task < 0) continue;
+        const GemmTask &k = s.tasks[t.task];
+        int64_t kp = 0;
+        for (int q = 0; q < k.seg_count; ++q) kp += (s.segs[k.seg_begin + q].K + 15) / 16 * 16;
+        padded += 2.0 * s.bm * s.bn * kp * (p.dtype == QLG_C128 ? 4 : 1);
+      }
+      std::fprintf(stderr, "[qlg plan] stage %d: tasks %zu segs %zu tiles %zu tile %dx%d gflop %.3f padded-eff %.3f\n",
+                   i + 1, s.tasks.size(), s.segs.size(), s.tiles.size(), s.bm, s.bn, s.flops / 1e9,
+                   padded > 0 ? s.flops / padded : 0.0);
+    }
+    std::fprintf(stderr, "[qlg plan] vector %lld elems, X %lld, Y %lld, C %lld, algo gflop %.3f\n",
+                 (long long)p.vec.size, (long long)p.x_elems, (long long)p.y_elems, (long long)p.c_elems,
+                 p.algo_flops / 1e9);
   }
   p.X.ensure(std::max<int64_t>(p.x_elems, 1) * ed * sizeof(double));
   p.Y.ensure(std::max<int64_t>(p.y_elems, 1) * ed * sizeof(double));
