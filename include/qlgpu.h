@@ -136,6 +136,27 @@ int qlg_plan_stage_flops(qlg_plan plan, double *flops3);
 /* out = H_eff * psi, both in the plan's vector layout (device). */
 int qlg_matvec(qlg_plan plan, const void *psi, void *out);
 
+/* ---- contractions around the solve (same grouped-GEMM machinery) ------------
+ * qlg_plan_contract: C = A x B over A's last k axes and B's first k axes
+ * (result axes: A's free axes then B's), with optional conjugation of either
+ * operand -- the reference's Contract(A, B, {{..}, {..}}, C) for the
+ * products it forms around the solve (psi0 = A_l . A_r,
+ * two_site_update_finite_vmps_impl.h:278-279; U.S and S.V^T, :322-331).
+ * qlg_plan_env: the environment updates UpdateSiteLenvs (side 'l') and
+ * UpdateSiteRenvs (side 'r'), include/qlmps/algorithm/vmps/vmps_init.h:343-373,
+ * from (env, mps tensor, mpo tensor).  The result layout (blocks in
+ * lexicographic order, prefix-sum offsets) is read with
+ * qlg_plan_vector_layout; result axes are [Inverse(A.b), W.w1, A.b] for 'l'
+ * and [Inverse(B.a), W.w0, B.a] for 'r'.
+ * qlg_execute runs such a plan: in0/in1 = (A, B) or (env, mps); NULL reuses
+ * the data pointers given at planning time; out is a caller buffer of the
+ * layout's size.                                                           */
+int qlg_plan_contract(qlg_ctx ctx, const qlg_bsten *A, const qlg_bsten *B, int k, int conj_a, int conj_b,
+                      qlg_plan *out);
+int qlg_plan_env(qlg_ctx ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side,
+                 qlg_plan *out);
+int qlg_execute(qlg_plan plan, const void *in0, const void *in1, void *out);
+
 /* ---- Krylov vector kernels (flat vectors of n elements) ------------------
  * qlten LinearCombine / Normalize / Dag+Contract equivalents,
  * lanczos_vmps_solver_impl.h:141,155-161,173-183,238.  Reductions are

@@ -1,0 +1,111 @@
+"""GPU parity of the contractions around the solve (tensordot, environment
+updates) against the oracle, and end-to-end two-site DMRG energies against the
+reference's known answers (tests/test_algorithm/test_finite_dmrg.cc:167-187,
+:729-791) and exact diagonalisation."""
+import numpy as np
+import pytest
+import torch
+
+from oracle.bsten import Contract
+from oracle.dmrg import UpdateSiteLenvs, UpdateSiteRenvs, init_envs, random_mps as o_random_mps
+from oracle.models import build_mpo as o_build_mpo, heisenberg_chain as o_heis, hubbard_cylinder as o_hub
+from tests.convert import to_device, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from ultradmrg_amd import mpo as M
+    from ultradmrg_amd.solver import LanczosParams
+    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, direct_state_mps, random_mps
+    from ultradmrg_amd.tensor import contract, env_update
+
+
+def rel_err(a, b):
+    assert sorted(a.blocks) == sorted(b.blocks), "block structure differs"
+    num = max(float(np.abs(a.blocks[k] - b.blocks[k]).max()) for k in a.blocks)
+    den = max(float(np.abs(b.blocks[k]).max()) for k in b.blocks)
+    return num / den
+
+
+def _problem(which):
+    if which == "heisenberg":
+        m, n, total = o_heis(10), 10, (0,)
+    else:
+        m, n, total = o_hub(3, 2), 6, (6, 0)
+    return m, n, o_build_mpo(m), o_random_mps(m, total, 24, seed=31)
+
+
+@pytest.mark.parametrize("which", ["heisenberg", "hubbard"])
+def test_tensordot_parity(which):
+    print(f"[case] tensordot {which}", flush=True)
+    m, n, mpo, mps = _problem(which)
+    i = n // 2 - 1
+    ref = Contract(mps[i], mps[i + 1], ([2], [0]))
+    out = contract(to_device(mps[i]), to_device(mps[i + 1]), 1)
+    assert out.coords == sorted(ref.blocks)
+    assert rel_err(to_oracle(out, ref), ref) < 1e-13
+
+
+@pytest.mark.parametrize("which", ["heisenberg", "hubbard"])
+def test_env_update_parity(which):
+    print(f"[case] env update {which}", flush=True)
+    m, n, mpo, mps = _problem(which)
+    lenvs, renvs = init_envs(mps, mpo)
+    L = lenvs[0]
+    dL = to_device(L)
+    for i in range(n // 2):
+        refL = UpdateSiteLenvs(L, mps[i], mpo[i])
+        dL = env_update(dL, to_device(mps[i]), to_device(mpo[i]), "l")
+        assert dL.coords == sorted(refL.blocks)
+        assert rel_err(to_oracle(dL, refL), refL) < 1e-12
+        L = refL
+    for i in range(1, n // 2):
+        refR = renvs[i]
+        dR = env_update(to_device(renvs[i - 1]), to_device(mps[n - i]), to_device(mpo[n - i]), "r")
+        assert dR.coords == sorted(refR.blocks)
+        assert rel_err(to_oracle(dR, refR), refR) < 1e-12
+
+
+def test_dmrg_heisenberg_n6_kat():
+    """test_finite_dmrg.cc:167-187: -2.493577133888 to 1e-12 (4 sweeps, D=8, LanczosParams(1e-7))."""
+    print("[case] dmrg heisenberg N=6", flush=True)
+    m = M.heisenberg_chain(6)
+    mps = direct_state_mps(m, [i % 2 for i in range(6)])
+    drv = TwoSiteVMPS(mps, M.build_mpo(m), SweepParams(4, 8, 8, 1e-9, LanczosParams(1e-7), verbose=False))
+    e = drv.run()
+    assert abs(e - (-2.493577133888)) < 1e-12
+
+
+def test_dmrg_hubbard_2x2_kat():
+    """test_finite_dmrg.cc:729-791 (t=1, t'=0.5, U=2): -2.828427124746 to 1e-10."""
+    print("[case] dmrg hubbard 2x2", flush=True)
+    m = M.Model(4, M.HUBBARD_QNS, M.hubbard_operators())
+    idx = lambda x, y: x * 2 + y
+    for i in range(2):
+        for j in range(2):
+            s0 = idx(i, j)
+            m.add(2.0, (s0, "Uterm"))
+            if i != 1:
+                M._hopping(m, 1.0, s0, idx(i + 1, j))
+            if j != 1:
+                M._hopping(m, 1.0, s0, idx(i, j + 1))
+                if i != 0:
+                    M._hopping(m, 0.5, s0, idx(i - 1, j + 1))
+                if i != 1:
+                    M._hopping(m, 0.5, s0, idx(i + 1, j + 1))
+    mps = direct_state_mps(m, [1 if i % 2 == 0 else 2 for i in range(4)])
+    drv = TwoSiteVMPS(mps, M.build_mpo(m), SweepParams(10, 16, 16, 1e-9, LanczosParams(1e-8, 20), verbose=False))
+    assert abs(drv.run() - (-2.828427124746)) < 1e-10
+
+
+def test_dmrg_config1_heisenberg_l20():
+    """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
+    truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""
+    print("[case] dmrg L=20 D=64", flush=True)
+    m = M.heisenberg_chain(20)
+    mps = random_mps(m, (0,), 64, seed=1001)
+    drv = TwoSiteVMPS(mps, M.build_mpo(m), SweepParams(4, 64, 64, 1e-10, LanczosParams(1e-10, 100),
+                                                       verbose=False))
+    e = drv.run()
+    assert abs(e - (-8.682473334398967)) < 1e-10 * 8.68
+    print(f"[config1] E0 {e:.15f} sweep seconds {[round(s, 3) for s in drv.stats.sweep_seconds]}", flush=True)

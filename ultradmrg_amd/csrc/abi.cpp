@@ -226,8 +226,40 @@ int qlg_matvec(qlg_plan plan, const void *psi, void *out) {
   if (!plan || !psi || !out) return QLG_ERR_INVALID;
   Plan &p = *plan->p;
   return guarded(p.ctx, [&] {
+    QLG_CHECK(p.kind == PLAN_HEFF, "qlg_matvec needs an effective-Hamiltonian plan");
     QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
     p.matvec(static_cast<const double *>(psi), static_cast<double *>(out));
+  });
+}
+
+int qlg_plan_contract(qlg_ctx ctx, const qlg_bsten *A, const qlg_bsten *B, int k, int conj_a, int conj_b,
+                      qlg_plan *out) {
+  if (!ctx || !out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    auto h = std::make_unique<qlg_plan_s>();
+    h->p.reset(build_tensordot_plan(&ctx->c, A, B, k, conj_a, conj_b));
+    *out = h.release();
+  });
+}
+
+int qlg_plan_env(qlg_ctx ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side,
+                 qlg_plan *out) {
+  if (!ctx || !out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    auto h = std::make_unique<qlg_plan_s>();
+    h->p.reset(build_env_plan(&ctx->c, env, mps, mpo, side));
+    *out = h.release();
+  });
+}
+
+int qlg_execute(qlg_plan plan, const void *in0, const void *in1, void *out) {
+  if (!plan || !out) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.kind != PLAN_HEFF, "use qlg_matvec for effective-Hamiltonian plans");
+    p.execute(in0, in1, out);
   });
 }
 

@@ -28,11 +28,15 @@ enum BaseSlot : uint8_t {
   NSLOTS = 8
 };
 
+// Segment flags: op(A) = A^T when A is stored [K x M] (row-major, lda);
+// op(B) = B^T when B is stored [N x K]; conjugation applies to complex data.
+enum SegFlags : uint8_t { SEG_TRANS_B = 1, SEG_TRANS_A = 2, SEG_CONJ_A = 4, SEG_CONJ_B = 8 };
+
 struct GemmSeg {
   int64_t a_off, b_off;
   int64_t lda, ldb;
   int32_t K;
-  uint8_t a_slot, b_slot, transB, pad;
+  uint8_t a_slot, b_slot, flags, pad;
 };
 
 struct GemmTask {

@@ -56,80 +56,61 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
 
   double ra[NP][A_PER], rb[NP][B_PER];
 
-  // load the BK slice starting at k0 of segment s into registers
+  // load the BK slice starting at k0 of segment s into registers; element e of
+  // a thread maps to (row, k) so that consecutive lanes read consecutive
+  // addresses of the operand's storage order
   auto load = [&](const GemmSeg &sg, int k0) {
     const double *A = slots.p[sg.a_slot] + sg.a_off * NP;
     const double *B = slots.p[sg.b_slot] + sg.b_off * NP;
+    const double sa = (sg.flags & SEG_CONJ_A) ? -1.0 : 1.0;
+    const double sb = (sg.flags & SEG_CONJ_B) ? -1.0 : 1.0;
+    const bool ta = sg.flags & SEG_TRANS_A, tb = sg.flags & SEG_TRANS_B;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      int e = tid + i * 256;
-      int m = e / BK, k = e % BK;
-      int gm = tl.m0 + m, gk = k0 + k;
+      const int e = tid + i * 256;
+      const int m = ta ? e % BM : e / BK, k = ta ? e / BM : e % BK;
+      const int gm = tl.m0 + m, gk = k0 + k;
       const bool ok = gm < M && gk < sg.K;
-      const int64_t ix = (int64_t)gm * sg.lda + gk;
+      const int64_t ix = ta ? (int64_t)gk * sg.lda + gm : (int64_t)gm * sg.lda + gk;
       if constexpr (CPLX) {
         const double2 v = ok ? reinterpret_cast<const double2 *>(A)[ix] : double2{0.0, 0.0};
         ra[0][i] = v.x;
-        ra[1][i] = v.y;
+        ra[1][i] = sa * v.y;
       } else {
         ra[0][i] = ok ? A[ix] : 0.0;
       }
     }
-    if (sg.transB) {
 #pragma unroll
-      for (int i = 0; i < B_PER; ++i) {
-        int e = tid + i * 256;
-        int n = e / BK, k = e % BK;
-        int gn = tl.n0 + n, gk = k0 + k;
-        const bool ok = gn < N && gk < sg.K;
-        const int64_t ix = (int64_t)gn * sg.ldb + gk;
-        if constexpr (CPLX) {
-          const double2 v = ok ? reinterpret_cast<const double2 *>(B)[ix] : double2{0.0, 0.0};
-          rb[0][i] = v.x;
-          rb[1][i] = v.y;
-        } else {
-          rb[0][i] = ok ? B[ix] : 0.0;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < B_PER; ++i) {
-        int e = tid + i * 256;
-        int k = e / BN, n = e % BN;
-        int gn = tl.n0 + n, gk = k0 + k;
-        const bool ok = gn < N && gk < sg.K;
-        const int64_t ix = (int64_t)gk * sg.ldb + gn;
-        if constexpr (CPLX) {
-          const double2 v = ok ? reinterpret_cast<const double2 *>(B)[ix] : double2{0.0, 0.0};
-          rb[0][i] = v.x;
-          rb[1][i] = v.y;
-        } else {
-          rb[0][i] = ok ? B[ix] : 0.0;
-        }
+    for (int i = 0; i < B_PER; ++i) {
+      const int e = tid + i * 256;
+      const int n = tb ? e / BK : e % BN, k = tb ? e % BK : e / BN;
+      const int gn = tl.n0 + n, gk = k0 + k;
+      const bool ok = gn < N && gk < sg.K;
+      const int64_t ix = tb ? (int64_t)gn * sg.ldb + gk : (int64_t)gk * sg.ldb + gn;
+      if constexpr (CPLX) {
+        const double2 v = ok ? reinterpret_cast<const double2 *>(B)[ix] : double2{0.0, 0.0};
+        rb[0][i] = v.x;
+        rb[1][i] = sb * v.y;
+      } else {
+        rb[0][i] = ok ? B[ix] : 0.0;
       }
     }
   };
-  auto store = [&](bool transB) {
+  auto store = [&](uint8_t flags) {
+    const bool ta = flags & SEG_TRANS_A, tb = flags & SEG_TRANS_B;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      int e = tid + i * 256;
+      const int e = tid + i * 256;
+      const int m = ta ? e % BM : e / BK, k = ta ? e / BM : e % BK;
 #pragma unroll
-      for (int p = 0; p < NP; ++p) As[p][e % BK][e / BK] = ra[p][i];
+      for (int p = 0; p < NP; ++p) As[p][k][m] = ra[p][i];
     }
-    if (transB) {
 #pragma unroll
-      for (int i = 0; i < B_PER; ++i) {
-        int e = tid + i * 256;
+    for (int i = 0; i < B_PER; ++i) {
+      const int e = tid + i * 256;
+      const int n = tb ? e / BK : e % BN, k = tb ? e % BK : e / BN;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) Bs[p][e % BK][e / BK] = rb[p][i];
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < B_PER; ++i) {
-        int e = tid + i * 256;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) Bs[p][e / BN][e % BN] = rb[p][i];
-      }
+      for (int p = 0; p < NP; ++p) Bs[p][k][n] = rb[p][i];
     }
   };
 
@@ -141,9 +122,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
     GemmSeg cur = segs[s];
     load(cur, 0);
     while (true) {
-      bool cur_trans = cur.transB;
+      const uint8_t cur_flags = cur.flags;
       __syncthreads();
-      store(cur_trans);
+      store(cur_flags);
       __syncthreads();
       // advance iterator and prefetch the next slice
       int ns = s, nk = k0 + BK;
@@ -156,8 +137,10 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
       }
       const bool more = ns < s_end;
       if (more) load(nxt, nk);
+      const int krem = cur.K - k0;  // valid k of the staged slice; the rest is zero fill
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
+        if (kk >= krem) break;
         double af[NP][FM], bf[NP][FN];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {

@@ -64,7 +64,7 @@ Ten parse_tensor(const qlg_bsten *d, int expect_rank, const char *name) {
   return t;
 }
 
-static void check_pair(const Axis &a, const Axis &b, const char *what) {
+void check_pair(const Axis &a, const Axis &b, const char *what) {
   QLG_CHECK(a.same_sectors(b) && a.dir == -b.dir, std::string("contracted axes do not match: ") + what);
 }
 
@@ -120,7 +120,7 @@ void Stage::upload(hipStream_t st) {
 
 // Bounds of every access a task list can make: rows < M, cols < N, k < K of
 // each segment, relative to the buffer behind its slot (in elements).
-static void validate_stage(const Stage &s, const int64_t *slot_elems) {
+void validate_stage(const Stage &s, const int64_t *slot_elems) {
   auto inside = [&](int slot, int64_t first, int64_t last, const char *what) {
     QLG_CHECK(slot > SLOT_ABS && slot < NSLOTS, std::string("task list: bad slot for ") + what);
     QLG_CHECK(first >= 0 && last < slot_elems[slot],
@@ -134,9 +134,14 @@ static void validate_stage(const Stage &s, const int64_t *slot_elems) {
     for (int q = 0; q < t.seg_count; ++q) {
       const GemmSeg &g = s.segs[t.seg_begin + q];
       if (g.K <= 0) continue;
-      QLG_CHECK(g.lda >= g.K, "task list: lda < K");
-      inside(g.a_slot, g.a_off, g.a_off + (int64_t)(t.M - 1) * g.lda + (g.K - 1), "A");
-      if (g.transB) {
+      if (g.flags & SEG_TRANS_A) {
+        QLG_CHECK(g.lda >= t.M, "task list: lda < M");
+        inside(g.a_slot, g.a_off, g.a_off + (int64_t)(g.K - 1) * g.lda + (t.M - 1), "A^T");
+      } else {
+        QLG_CHECK(g.lda >= g.K, "task list: lda < K");
+        inside(g.a_slot, g.a_off, g.a_off + (int64_t)(t.M - 1) * g.lda + (g.K - 1), "A");
+      }
+      if (g.flags & SEG_TRANS_B) {
         QLG_CHECK(g.ldb >= g.K, "task list: ldb < K");
         inside(g.b_slot, g.b_off, g.b_off + (int64_t)(t.N - 1) * g.ldb + (g.K - 1), "B^T");
       } else {
@@ -150,7 +155,7 @@ static void validate_stage(const Stage &s, const int64_t *slot_elems) {
 // Pick the FP64 tile shape minimising a simple cost model: MFMA time of the
 // padded tiles spread over 256 CUs (2 workgroups each), bounded below by the
 // longest single tile, plus a fixed per-tile cost.
-static void autotune_tiles(Stage &s, bool cplx) {
+void autotune_tiles(Stage &s, bool cplx) {
   static const int shapes[][2] = {{32, 32}, {64, 32}, {32, 64}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
   const int nshapes = cplx ? 4 : 7;   // complex tiles keep two accumulator planes: cap at 64x64
   double best = 1e300;
@@ -573,7 +578,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       sg.b_off = p.vec.off[ch.vblk] + (ch.sl[0] * d1 + ch.sl[1]) * Db;
       sg.ldb = d0 * d1 * Db;
       sg.K = (int32_t)Da;
-      sg.transB = 0;
+      sg.flags = 0;
       GemmTask tk{};
       tk.c_slot = SLOT_X;
       tk.c_off = c.x_off + (int64_t)q * c.P;
@@ -642,7 +647,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
               sg.b_slot = SLOT_R;
               sg.b_off = R.off[rb] + zz * Db;
               sg.ldb = Wz * Db;
-              sg.transB = 1;
+              sg.flags = SEG_TRANS_B;
               sg.K = (int32_t)Db;
               s3.segs.push_back(sg);
               tk.seg_count++;

@@ -44,8 +44,12 @@ struct Layout {
   }
 };
 
+enum PlanKind { PLAN_HEFF = 0, PLAN_TENSORDOT = 1, PLAN_ENV_LEFT = 2, PLAN_ENV_RIGHT = 3 };
+
 struct Plan {
   Ctx *ctx = nullptr;
+  int kind = PLAN_HEFF;
+  int nstages = 3;
   int nphys = 2;
   int dtype = QLG_F64;
   int precision = QLG_PREC_F64;
@@ -63,10 +67,17 @@ struct Plan {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Plan();
   void matvec(const double *in, double *out);
+  void execute(const void *in0, const void *in1, void *out);
 };
 
 Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
                  const qlg_bsten *psi, int precision);
+Plan *build_tensordot_plan(Ctx *ctx, const qlg_bsten *A, const qlg_bsten *B, int k, int conj_a, int conj_b);
+Plan *build_env_plan(Ctx *ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side);
+
+void check_pair(const Axis &a, const Axis &b, const char *what);
+void validate_stage(const Stage &s, const int64_t *slot_elems);
+void autotune_tiles(Stage &s, bool cplx);
 
 void launch_grouped_gemm_f64(int bm, int bn, bool cplx, const GemmTile *tiles, const GemmTask *tasks,
                              const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st);

@@ -226,3 +226,66 @@ def linear_combine(coefs, bases: Sequence[torch.Tensor], beta, y: torch.Tensor, 
     ptrs = (C.c_void_p * max(k, 1))(*[b.data_ptr() for b in bases])
     torch.cuda.synchronize()
     ctx.check(ctx.lib.qlg_linear_combine(ctx.h, dtype_code(y), y.numel(), k, cf, ptrs, _scalar(beta, cplx), _ptr(y)))
+
+
+# ---------------------------------------------------------------------------
+# contractions around the Krylov solve
+# ---------------------------------------------------------------------------
+class _OpPlan:
+    """A tensordot / environment-update plan and its result layout."""
+
+    def __init__(self, ctx: Context, handle, axes, rank):
+        self.ctx, self.h, self.axes = ctx, handle, axes
+        lib = ctx.lib
+        nb, size = C.c_int64(), C.c_int64()
+        ctx.check(lib.qlg_plan_vector_layout(handle, C.byref(nb), C.byref(size), None, None))
+        coords = (C.c_int32 * max(nb.value * rank, 1))()
+        offs = (C.c_int64 * max(nb.value, 1))()
+        ctx.check(lib.qlg_plan_vector_layout(handle, None, None, coords, offs))
+        self.coords = [tuple(coords[b * rank:(b + 1) * rank]) for b in range(nb.value)]
+        self.offsets = list(offs[:nb.value])
+        self.size = size.value
+
+    def run(self, in0: BlockTensor, in1: BlockTensor, dtype) -> BlockTensor:
+        out = torch.empty(max(self.size, 1), dtype=dtype, device=in0.data.device)
+        self.ctx.check(self.ctx.lib.qlg_execute(self.h, C.c_void_p(in0.data.data_ptr()),
+                                                C.c_void_p(in1.data.data_ptr()), C.c_void_p(out.data_ptr())))
+        self.ctx.synchronize()
+        return BlockTensor(self.axes, self.coords, out, self.offsets)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.qlg_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+def contract(A: BlockTensor, B: BlockTensor, k: int = 1, conj_a=False, conj_b=False,
+             ctx: Context = None) -> BlockTensor:
+    """C = A x B over A's last k axes and B's first k axes (result: A free axes, then B's)."""
+    ctx = ctx or default_context(A.data.device.index or 0)
+    h = C.c_void_p()
+    da, db = A.descriptor(), B.descriptor()
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.qlg_plan_contract(ctx.h, C.byref(da), C.byref(db), k, int(conj_a), int(conj_b), C.byref(h)))
+    axes = A.axes[:A.rank - k] + B.axes[k:]
+    plan = _OpPlan(ctx, h, axes, len(axes))
+    dtype = torch.complex128 if (A.is_complex or B.is_complex) else torch.float64
+    return plan.run(A, B, dtype)
+
+
+def env_update(env: BlockTensor, mps: BlockTensor, mpo: BlockTensor, side: str,
+               ctx: Context = None) -> BlockTensor:
+    """UpdateSiteLenvs (side 'l') / UpdateSiteRenvs (side 'r') on the GPU."""
+    ctx = ctx or default_context(env.data.device.index or 0)
+    h = C.c_void_p()
+    de, dm, dw = env.descriptor(), mps.descriptor(), mpo.descriptor()
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.qlg_plan_env(ctx.h, C.byref(de), C.byref(dm), C.byref(dw), side.encode(), C.byref(h)))
+    if side == "l":
+        axes = [mps.axes[2].inverse(), mpo.axes[3], mps.axes[2]]
+    else:
+        axes = [mps.axes[0].inverse(), mpo.axes[0], mps.axes[0]]
+    plan = _OpPlan(ctx, h, axes, 3)
+    return plan.run(env, mps, env.data.dtype)
