@@ -62,6 +62,68 @@ class SVDResult:
     entropy: float
 
 
+def _sector_svds(mats) -> dict:
+    """Thin SVDs of the per-sector matrices through the eigen-decomposition of
+    the Gram matrix on each matrix's smaller side, batched on the GPU by size
+    bucket (rocSOLVER's batched symmetric eigensolver; a per-matrix loop is
+    latency bound for the ~100 small sectors of a D=2000 state).  Padded rows
+    and columns carry eigenvalue -1 so they separate from the non-negative
+    spectrum.  Returns q -> (U, S, Vh, rks, cks, ro, co, g) with S descending;
+    singular vectors of (near-)zero singular values are completed to an
+    orthonormal set."""
+    buckets: Dict[int, list] = {}
+    for q, (M, *_rest) in mats.items():
+        n = min(M.shape)
+        b = 128
+        while b < n:
+            b *= 2
+        buckets.setdefault(b, []).append(q)
+    out = {}
+    for b, qs in buckets.items():
+        dev, dt = mats[qs[0]][0].device, mats[qs[0]][0].dtype
+        G = torch.zeros((len(qs), b, b), dtype=dt, device=dev)
+        for j, q in enumerate(qs):
+            M = mats[q][0]
+            n = min(M.shape)
+            G[j, :n, :n] = M @ M.mT.conj() if M.shape[0] <= M.shape[1] else M.mT.conj() @ M
+            if n < b:
+                G[j, n:, n:] = -torch.eye(b - n, dtype=dt, device=dev)
+        w, V = torch.linalg.eigh(G)
+        for j, q in enumerate(qs):
+            M, rks, cks, ro, co, g = mats[q]
+            n = min(M.shape)
+            lam = w[j, b - n:].flip(0).clamp_min(0.0)           # descending
+            vec = V[j, :n, b - n:].flip(1)
+            S = lam.sqrt().to(torch.float64)
+            tiny = S <= 1e-14 * max(float(S[0]), 1e-300)
+            Sinv = torch.where(tiny, torch.zeros_like(S), 1.0 / torch.where(tiny, torch.ones_like(S), S))
+            if M.shape[0] <= M.shape[1]:
+                U = vec
+                Vh = (U.mT.conj() @ M) * Sinv[:, None].to(dt)
+                if bool(tiny.any()):
+                    Vh = _complete_rows(Vh, tiny)
+            else:
+                Vm = vec
+                U = (M @ Vm) * Sinv[None, :].to(dt)
+                if bool(tiny.any()):
+                    U = _complete_rows(U.mT, tiny).mT
+                Vh = Vm.mT.conj()
+            out[q] = (U, S, Vh, rks, cks, ro, co, g)
+    return out
+
+
+def _complete_rows(A: torch.Tensor, bad: torch.Tensor) -> torch.Tensor:
+    """Replace the rows of A flagged in ``bad`` by unit vectors orthogonal to the others."""
+    k, n = A.shape
+    good = A[~bad]
+    gen = torch.Generator(device="cpu").manual_seed(1234)
+    R = torch.randn(int(bad.sum()), n, generator=gen, dtype=torch.float64).to(A.device, A.dtype)
+    Q, _ = torch.linalg.qr(torch.cat([good, R]).mT)
+    out = A.clone()
+    out[bad] = Q.mT[good.shape[0]:]
+    return out
+
+
 def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dmax: int) -> SVDResult:
     left, right = t.axes[:ldims], t.axes[ldims:]
     ldiv = tuple(ldiv)
@@ -77,8 +139,7 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         g["rows"].setdefault(lk, int(np.prod([ax.degs[s] for ax, s in zip(left, lk)])))
         g["cols"].setdefault(rk, int(np.prod([ax.degs[s] for ax, s in zip(right, rk)])))
         g["blocks"].append((i, lk, rk))
-    dec = {}
-    svals = []
+    mats = {}
     for q in sorted(groups):
         g = groups[q]
         rks, cks = sorted(g["rows"]), sorted(g["cols"])
@@ -97,9 +158,9 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         for i, lk, rk in g["blocks"]:
             M[ro[lk]:ro[lk] + g["rows"][lk], co[rk]:co[rk] + g["cols"][rk]] = \
                 block_view(t, i).reshape(g["rows"][lk], g["cols"][rk])
-        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
-        dec[q] = (U, S, Vh, rks, cks, ro, co, g)
-        svals.append(S)
+        mats[q] = (M, rks, cks, ro, co, g)
+    dec = _sector_svds(mats)
+    svals = [dec[q][1] for q in sorted(groups)]
     sv = torch.cat(svals).double().cpu().numpy()
     owner = np.concatenate([[qi] * len(dec[q][1]) for qi, q in enumerate(sorted(groups))])
     order = np.argsort(-sv, kind="stable")
