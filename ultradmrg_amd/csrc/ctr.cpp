@@ -25,12 +25,6 @@ namespace qlg {
 
 namespace {
 
-std::vector<int64_t> sector_offsets(const Axis &a) {
-  std::vector<int64_t> o(a.deg.size() + 1, 0);
-  for (size_t s = 0; s < a.deg.size(); ++s) o[s + 1] = o[s] + a.deg[s];
-  return o;
-}
-
 int64_t extent(const Ten &t) {
   int64_t e = 0;
   for (size_t b = 0; b < t.blk.size(); ++b) e = std::max(e, t.off[b] + t.blk_elems(b));

@@ -39,12 +39,40 @@ struct GemmSeg {
   uint8_t a_slot, b_slot, flags, pad;
 };
 
+// c_rowtab != 0: output row r of the task is written at element
+// crow_table[c_row + r] of the C slot instead of c_off + r * ldc.
 struct GemmTask {
   int64_t c_off;
   int64_t ldc;
   int32_t M, N;
   int32_t seg_begin, seg_count;
-  uint8_t c_slot, pad[7];
+  int32_t c_row;
+  uint8_t c_slot, c_rowtab, pad[2];
+};
+
+// Stage-2 (MPO) channel: K_out x K_in dense operator at c_off in the
+// coefficient buffer; its input / output rows are RowDesc entries starting
+// at in_rows / out_rows; positions n = a' * Db + b for n < P.
+struct ChanTask {
+  int64_t c_off;
+  int32_t kin, kout;
+  int32_t P, Db;
+  int32_t in_rows, out_rows;
+};
+
+// input rows of one channel are staged in LDS by the MPO-stage kernel
+constexpr int kMaxChanIn = 1024;
+
+struct RowDesc {
+  int64_t off;   // element offset of position (0, 0)
+  int64_t rs;    // element stride between consecutive a'
+};
+
+// 2-D block copy (element offsets and leading dimensions)
+struct CopyDesc {
+  int64_t src, dst;
+  int64_t sld, dld;
+  int32_t rows, cols;
 };
 
 struct GemmTile {
@@ -55,6 +83,7 @@ struct GemmTile {
 
 struct Slots {
   double *p[NSLOTS];
+  const int64_t *crows;   // row-address table of tasks with c_rowtab set
 };
 
 }  // namespace qlg

@@ -1,9 +1,17 @@
-// Persistent-free grouped FP64 GEMM on CDNA4 MFMA (v_mfma_f64_16x16x4_f64).
+// Grouped FP64 GEMM on CDNA4 MFMA (v_mfma_f64_16x16x4_f64).
 //
 // One workgroup (4 wavefronts, 2x2) computes one BM x BN output tile of one
-// task; it walks the task's K-segments in order, staging BK-deep slices of A
-// and B through LDS with a register prefetch of the next slice.  Each wave
-// owns a (BM/2) x (BN/2) sub-tile made of 16x16 MFMA blocks.
+// task and walks the task's K-segments in order.  BK-deep slices of A and B
+// are staged through two LDS buffers (ping-pong): the raw buffer loads of
+// slice i+1 are issued before the MFMAs of slice i and land in registers,
+// which are written to the other LDS buffer afterwards, so each K step costs
+// one barrier and the load latency hides behind the MFMA work.  Operands are
+// read with raw buffer loads whose out-of-range elements come back as zero
+// (tile_io.h), so the loads carry no branch and no select.  Slices run as
+// straight-line code, which lets the LDS fragment reads of one 4-deep
+// sub-step overlap the MFMAs of the previous one (a segment's last slice is
+// zero-filled past K and runs the same code).  Each wave owns a
+// (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks.
 //
 // MFMA f64 16x16x4 operand layout (one double per lane):
 //   A: lane l holds A[row = l % 16][k = l / 16]
@@ -14,34 +22,36 @@
 
 #include "common.h"
 #include "gemm.h"
+#include "tile_io.h"
 
 namespace qlg {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-constexpr int BK = 16;
+namespace {
 constexpr int PAD = 1;
+}
 
-template <int BM, int BN, bool CPLX>
+template <int BM, int BN, int BK, bool CPLX>
 __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *__restrict__ tiles,
                                                                const GemmTask *__restrict__ tasks,
                                                                const GemmSeg *__restrict__ segs,
                                                                Slots slots, int ntiles) {
   constexpr int WM = BM / 2, WN = BN / 2;       // wave sub-tile
   constexpr int FM = WM / 16, FN = WN / 16;     // MFMA blocks per wave
-  constexpr int A_PER = BM * BK / 256;          // A elements loaded per thread
+  constexpr int A_PER = BM * BK / 256;          // A elements loaded per thread per slice
   constexpr int B_PER = BK * BN / 256;
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small for 256 threads");
   constexpr int NP = CPLX ? 2 : 1;              // real / imaginary planes
-  __shared__ double As[NP][BK][BM + PAD];
-  __shared__ double Bs[NP][BK][BN + PAD];
+  constexpr uint32_t EB = CPLX ? 16u : 8u;      // element bytes
+  __shared__ double As[2][NP][BK][BM + PAD];
+  __shared__ double Bs[2][NP][BK][BN + PAD];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
-  const int tile_id = blockIdx.x;
-  if (tile_id >= ntiles) return;
-  const GemmTile tl = tiles[tile_id];
+  if ((int)blockIdx.x >= ntiles) return;
+  const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;  // padding slot of the XCD-balanced launch order
   const GemmTask tk = tasks[tl.task];
   const int M = tk.M, N = tk.N;
@@ -56,77 +66,108 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
 
   double ra[NP][A_PER], rb[NP][B_PER];
 
-  // load the BK slice starting at k0 of segment s into registers; element e of
-  // a thread maps to (row, k) so that consecutive lanes read consecutive
-  // addresses of the operand's storage order
+  // element e of a slice: (m, k) of A and (k, n) of B, chosen so that
+  // consecutive threads read consecutive addresses in either storage order.
+  // The transposition flags are uniform, so the choice is a select on the
+  // indices and every slice issues one straight sequence of loads.
+  auto a_mk = [&](int i, bool ta, int &m, int &k) {
+    const int e = tid + i * 256;
+    m = ta ? e % BM : e / BK;
+    k = ta ? e / BM : e % BK;
+  };
+  auto b_kn = [&](int i, bool tb, int &k, int &n) {
+    const int e = tid + i * 256;
+    n = tb ? e / BK : e % BN;
+    k = tb ? e % BK : e / BN;
+  };
   auto load = [&](const GemmSeg &sg, int k0) {
-    const double *A = slots.p[sg.a_slot] + sg.a_off * NP;
-    const double *B = slots.p[sg.b_slot] + sg.b_off * NP;
-    const double sa = (sg.flags & SEG_CONJ_A) ? -1.0 : 1.0;
-    const double sb = (sg.flags & SEG_CONJ_B) ? -1.0 : 1.0;
+    const __amdgpu_buffer_rsrc_t ar = make_rsrc(slots.p[sg.a_slot] + sg.a_off * NP);
+    const __amdgpu_buffer_rsrc_t br = make_rsrc(slots.p[sg.b_slot] + sg.b_off * NP);
+    const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
     const bool ta = sg.flags & SEG_TRANS_A, tb = sg.flags & SEG_TRANS_B;
+    const int K = sg.K;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256;
-      const int m = ta ? e % BM : e / BK, k = ta ? e / BM : e % BK;
-      const int gm = tl.m0 + m, gk = k0 + k;
-      const bool ok = gm < M && gk < sg.K;
-      const int64_t ix = ta ? (int64_t)gk * sg.lda + gm : (int64_t)gm * sg.lda + gk;
+      int m, k;
+      a_mk(i, ta, m, k);
+      const uint32_t gm = tl.m0 + m, gk = k0 + k;
+      const uint32_t ix = ta ? gk * lda + gm : gm * lda + gk;
+      const uint32_t off = ((int)gm < M && (int)gk < K) ? ix * EB : kOob;
       if constexpr (CPLX) {
-        const double2 v = ok ? reinterpret_cast<const double2 *>(A)[ix] : double2{0.0, 0.0};
+        const double2 v = load_c128(ar, off);
         ra[0][i] = v.x;
-        ra[1][i] = sa * v.y;
+        ra[1][i] = v.y;
       } else {
-        ra[0][i] = ok ? A[ix] : 0.0;
+        ra[0][i] = load_f64(ar, off);
       }
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      const int e = tid + i * 256;
-      const int n = tb ? e / BK : e % BN, k = tb ? e % BK : e / BN;
-      const int gn = tl.n0 + n, gk = k0 + k;
-      const bool ok = gn < N && gk < sg.K;
-      const int64_t ix = tb ? (int64_t)gn * sg.ldb + gk : (int64_t)gk * sg.ldb + gn;
+      int k, n;
+      b_kn(i, tb, k, n);
+      const uint32_t gn = tl.n0 + n, gk = k0 + k;
+      const uint32_t ix = tb ? gn * ldb + gk : gk * ldb + gn;
+      const uint32_t off = ((int)gn < N && (int)gk < K) ? ix * EB : kOob;
       if constexpr (CPLX) {
-        const double2 v = ok ? reinterpret_cast<const double2 *>(B)[ix] : double2{0.0, 0.0};
+        const double2 v = load_c128(br, off);
         rb[0][i] = v.x;
-        rb[1][i] = sb * v.y;
+        rb[1][i] = v.y;
       } else {
-        rb[0][i] = ok ? B[ix] : 0.0;
+        rb[0][i] = load_f64(br, off);
       }
     }
   };
-  auto store = [&](uint8_t flags) {
+  auto store = [&](int buf, uint8_t flags) {
     const bool ta = flags & SEG_TRANS_A, tb = flags & SEG_TRANS_B;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256;
-      const int m = ta ? e % BM : e / BK, k = ta ? e / BM : e % BK;
-#pragma unroll
-      for (int p = 0; p < NP; ++p) As[p][k][m] = ra[p][i];
+      int m, k;
+      a_mk(i, ta, m, k);
+      As[buf][0][k][m] = ra[0][i];
+      if constexpr (CPLX) As[buf][1][k][m] = (flags & SEG_CONJ_A) ? -ra[1][i] : ra[1][i];
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      const int e = tid + i * 256;
-      const int n = tb ? e / BK : e % BN, k = tb ? e % BK : e / BN;
-#pragma unroll
-      for (int p = 0; p < NP; ++p) Bs[p][k][n] = rb[p][i];
+      int k, n;
+      b_kn(i, tb, k, n);
+      Bs[buf][0][k][n] = rb[0][i];
+      if constexpr (CPLX) Bs[buf][1][k][n] = (flags & SEG_CONJ_B) ? -rb[1][i] : rb[1][i];
     }
+  };
+  // one 4-deep sub-step of the staged slice
+  auto step = [&](int buf, int kk) {
+    double af[NP][FM], bf[NP][FN];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[p][i] = As[buf][p][kk + (lane >> 4)][wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[p][j] = Bs[buf][p][kk + (lane >> 4)][wn0 + j * 16 + (lane & 15)];
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], bf[0][j], acc[0][i][j], 0, 0, 0);
+        if constexpr (CPLX) {
+          acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[1][i], bf[1][j], acc[0][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], bf[1][j], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[1][i], bf[0][j], acc[1][i][j], 0, 0, 0);
+        }
+      }
   };
 
   const int s_end = tk.seg_begin + tk.seg_count;
-  int s = tk.seg_begin, k0 = 0;
-  // skip empty segments
+  int s = tk.seg_begin;
   while (s < s_end && segs[s].K <= 0) ++s;
   if (s < s_end) {
     GemmSeg cur = segs[s];
+    int k0 = 0, buf = 0;
     load(cur, 0);
+    store(0, cur.flags);
+    __syncthreads();
     while (true) {
-      const uint8_t cur_flags = cur.flags;
-      __syncthreads();
-      store(cur_flags);
-      __syncthreads();
-      // advance iterator and prefetch the next slice
+      // iterator of the next slice; its loads stay in flight during this slice's MFMAs
       int ns = s, nk = k0 + BK;
       GemmSeg nxt = cur;
       if (nk >= cur.K) {
@@ -137,37 +178,36 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
       }
       const bool more = ns < s_end;
       if (more) load(nxt, nk);
-      const int krem = cur.K - k0;  // valid k of the staged slice; the rest is zero fill
+      // the staged slice is zero-filled past the segment's K, so a partial
+      // slice runs the same straight-line code as a full one
 #pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        if (kk >= krem) break;
-        double af[NP][FM], bf[NP][FN];
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-#pragma unroll
-          for (int i = 0; i < FM; ++i) af[p][i] = As[p][kk + (lane >> 4)][wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-          for (int j = 0; j < FN; ++j) bf[p][j] = Bs[p][kk + (lane >> 4)][wn0 + j * 16 + (lane & 15)];
-        }
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], bf[0][j], acc[0][i][j], 0, 0, 0);
-            if constexpr (CPLX) {
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[1][i], bf[1][j], acc[0][i][j], 0, 0, 0);
-              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], bf[1][j], acc[1][i][j], 0, 0, 0);
-              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[1][i], bf[0][j], acc[1][i][j], 0, 0, 0);
-            }
-          }
-      }
+      for (int kk = 0; kk < BK; kk += 4) step(buf, kk);
       if (!more) break;
+      store(buf ^ 1, nxt.flags);
+      __syncthreads();
+      buf ^= 1;
       s = ns;
       k0 = nk;
       cur = nxt;
     }
   }
 
+  // output row addresses: a row-address table or a plain row stride
+  int64_t crow[FM][4];
+  if (tk.c_rowtab) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.m0 + wm0 + i * 16 + (lane >> 4) + 4 * r;
+        crow[i][r] = slots.crows[tk.c_row + min(row, M - 1)] - tk.c_off;
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) crow[i][r] = (int64_t)(tl.m0 + wm0 + i * 16 + (lane >> 4) + 4 * r) * tk.ldc;
+  }
   double *C = slots.p[tk.c_slot] + tk.c_off * NP;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -178,7 +218,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
       for (int r = 0; r < 4; ++r) {
         const int row = tl.m0 + wm0 + i * 16 + (lane >> 4) + 4 * r;
         if (row < M && col < N) {
-          const int64_t ix = (int64_t)row * tk.ldc + col;
+          const int64_t ix = crow[i][r] + col;
           if constexpr (CPLX)
             reinterpret_cast<double2 *>(C)[ix] = double2{acc[0][i][j][r], acc[1][i][j][r]};
           else
@@ -189,33 +229,50 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
 }
 
 template <bool CPLX>
-static void launch_shape(int bm, int bn, const GemmTile *tiles, const GemmTask *tasks, const GemmSeg *segs,
+static void launch_shape(int bm, int bn, int bk, const GemmTile *tiles, const GemmTask *tasks, const GemmSeg *segs,
                          const Slots &slots, int ntiles, hipStream_t st) {
   dim3 grid(ntiles), block(256);
-#define QLG_SHAPE(M_, N_)                                                                                    \
-  if (bm == M_ && bn == N_) {                                                                                \
-    hipLaunchKernelGGL((grouped_gemm_f64_kernel<M_, N_, CPLX>), grid, block, 0, st, tiles, tasks, segs, slots, \
-                       ntiles);                                                                              \
+#define QLG_SHAPE(M_, N_, K_)                                                                                \
+  if (bm == M_ && bn == N_ && bk == K_) {                                                                    \
+    hipLaunchKernelGGL((grouped_gemm_f64_kernel<M_, N_, K_, CPLX>), grid, block, 0, st, tiles, tasks, segs,   \
+                       slots, ntiles);                                                                       \
     return;                                                                                                  \
   }
-  QLG_SHAPE(32, 32)
-  QLG_SHAPE(64, 32)
-  QLG_SHAPE(32, 64)
-  QLG_SHAPE(64, 64)
-  QLG_SHAPE(128, 64)
-  QLG_SHAPE(64, 128)
-  QLG_SHAPE(128, 128)
+  QLG_SHAPE(32, 32, 16)
+  QLG_SHAPE(64, 32, 16)
+  QLG_SHAPE(32, 64, 16)
+  QLG_SHAPE(64, 64, 16)
+  QLG_SHAPE(128, 64, 16)
+  QLG_SHAPE(64, 128, 16)
+  QLG_SHAPE(128, 128, 16)
+  if constexpr (!CPLX) {  // two accumulator planes of complex tiles do not fit BK=32 double buffers
+    QLG_SHAPE(32, 32, 32)
+    QLG_SHAPE(64, 32, 32)
+    QLG_SHAPE(32, 64, 32)
+    QLG_SHAPE(64, 64, 32)
+    QLG_SHAPE(128, 64, 32)
+    QLG_SHAPE(64, 128, 32)
+  }
 #undef QLG_SHAPE
   throw Error(QLG_ERR_UNSUPPORTED, "unsupported FP64 tile shape");
+}
+
+int gemm_bk() {
+  static int bk = [] {
+    const char *v = std::getenv("QLG_BK");
+    return (v && std::atoi(v) == 32) ? 32 : 16;
+  }();
+  return bk;
 }
 
 void launch_grouped_gemm_f64(int bm, int bn, bool cplx, const GemmTile *tiles, const GemmTask *tasks,
                              const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st) {
   if (ntiles <= 0) return;
+  const int bk = (cplx || bm * bn > 64 * 128) ? 16 : gemm_bk();
   if (cplx)
-    launch_shape<true>(bm, bn, tiles, tasks, segs, slots, ntiles, st);
+    launch_shape<true>(bm, bn, bk, tiles, tasks, segs, slots, ntiles, st);
   else
-    launch_shape<false>(bm, bn, tiles, tasks, segs, slots, ntiles, st);
+    launch_shape<false>(bm, bn, bk, tiles, tasks, segs, slots, ntiles, st);
   QLG_HIP(hipGetLastError());
 }
 

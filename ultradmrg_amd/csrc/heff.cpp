@@ -1,0 +1,677 @@
+// Effective-Hamiltonian planner (two-site and one-site).
+//
+// The sector pairing is done once per plan and turned into three launches:
+//  stage 1  per ket sector a of the state:  T1_a = L_a . Psi_a, where L_a stacks
+//           every L block (a', w, a) (rows (a', w)) and Psi_a is the state's
+//           fused matrix [D_a x N_a] (all blocks (a, s1, s2, b) side by side);
+//  stage 2  per channel (a'-sector, b-sector): Y = C . X on rows gathered from
+//           T1 (one per (w, s1, s2) component) and scattered into Y, C being the
+//           combined MPO operator sum_y Wl[w,s1,t1,y] Wr[y,s2,t2,z];
+//  stage 3  per bra sector b': out = Y_b' . R_b'^T, where R_b' holds every R
+//           block (b', z, b) side by side (one long K) and Y_b' the rows of all
+//           output blocks with bra b'; output rows land in the fused layout.
+// The Krylov vectors inside the library use the fused layout; the boundary
+// keeps the canonical one (pack / unpack copies at the solver's entry / exit).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <set>
+
+#include "plan.h"
+
+namespace qlg {
+
+Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *const *Wd, const qlg_bsten *Rd,
+                 const qlg_bsten *psid, int precision) {
+  QLG_CHECK(nphys == 1 || nphys == 2, "nphys must be 1 or 2");
+  std::unique_ptr<Plan> P(new Plan());
+  Plan &p = *P;
+  p.ctx = ctx;
+  p.nphys = nphys;
+  p.precision = precision;
+  p.L = parse_tensor(Ld, 3, "L");
+  for (int k = 0; k < nphys; ++k) p.W[k] = parse_tensor(Wd[k], 4, k == 0 ? "Wl" : "Wr");
+  p.R = parse_tensor(Rd, 3, "R");
+  Ten psi = parse_tensor(psid, 2 + nphys, "psi");
+  p.dtype = psi.dtype;
+  const int qn_len = psi.qn_len;
+  for (const Ten *t : {&p.L, &p.W[0], &p.W[nphys - 1], &p.R}) {
+    QLG_CHECK(t->dtype == p.dtype, "all operands must share one dtype");
+    QLG_CHECK(t->qn_len == qn_len, "all operands must share one quantum-number length");
+  }
+  const Ten &L = p.L, &R = p.R;
+  const Ten &W0 = p.W[0], &W1 = p.W[nphys - 1];
+  // contracted-axis compatibility (reference index orders)
+  check_pair(L.ax[2], psi.ax[0], "L ket / psi left");
+  check_pair(L.ax[1], W0.ax[0], "L mpo / W left");
+  for (int k = 0; k < nphys; ++k) check_pair(p.W[k].ax[1], psi.ax[1 + k], "W s_in / psi physical");
+  if (nphys == 2) check_pair(W0.ax[3], W1.ax[0], "Wl right / Wr left");
+  check_pair(W1.ax[3], R.ax[1], "W right / R mpo");
+  check_pair(R.ax[2], psi.ax[1 + nphys], "R ket / psi right");
+  // result axes must be the state's axes so that H_eff maps the space into itself
+  {
+    std::vector<const Axis *> out_ax = {&L.ax[0]};
+    for (int k = 0; k < nphys; ++k) out_ax.push_back(&p.W[k].ax[2]);
+    out_ax.push_back(&R.ax[0]);
+    for (int i = 0; i < psi.rank; ++i)
+      QLG_CHECK(out_ax[i]->same_sectors(psi.ax[i]) && out_ax[i]->dir == psi.ax[i].dir,
+                "H_eff output axis " + std::to_string(i) + " differs from the state's axis");
+  }
+  const int ed = elem_doubles(p.dtype);
+  const double fl = p.dtype == QLG_C128 ? 8.0 : 2.0;
+  const int rk = psi.rank;
+  const int bax = rk - 1;  // right bond axis of psi
+
+  // ---- indexes of operator blocks ------------------------------------------
+  std::map<int, std::vector<int64_t>> L_by_ket;             // a -> L blocks
+  for (size_t b = 0; b < L.blk.size(); ++b) L_by_ket[L.blk[b][2]].push_back(b);
+  std::map<std::pair<int, int>, std::vector<int64_t>> W_by_in[2];   // (wl, s_in) -> W blocks
+  for (int k = 0; k < nphys; ++k)
+    for (size_t b = 0; b < p.W[k].blk.size(); ++b)
+      W_by_in[k][{p.W[k].blk[b][0], p.W[k].blk[b][1]}].push_back(b);
+  std::map<std::pair<int, int>, std::vector<int64_t>> R_by_zb;    // (z, b) -> R blocks
+  std::map<int, std::vector<int64_t>> R_by_bra;                  // b' -> R blocks
+  for (size_t b = 0; b < R.blk.size(); ++b) {
+    R_by_zb[{R.blk[b][1], R.blk[b][2]}].push_back(b);
+    R_by_bra[R.blk[b][0]].push_back(b);
+  }
+
+  // ---- symbolic reference schedule: closure of the block set + FLOPs --------
+  auto symbolic = [&](const std::set<Coords> &in, double *flops) {
+    std::set<Coords> out;
+    double f = 0;
+    // stage T1: (a', w, s..., b)
+    std::set<Coords> t1;
+    for (const Coords &c : in) {
+      auto it = L_by_ket.find(c[0]);
+      if (it == L_by_ket.end()) continue;
+      int64_t nfree = 1;
+      for (int i = 1; i < rk; ++i) nfree *= psi.ax[i].deg[c[i]];
+      for (int64_t lb : it->second) {
+        const Coords &l = L.blk[lb];
+        Coords t{};
+        t[0] = l[0];
+        t[1] = l[1];
+        for (int i = 1; i < rk; ++i) t[1 + i] = c[i];
+        t1.insert(t);
+        f += fl * double(L.ax[0].deg[l[0]] * L.ax[1].deg[l[1]]) * double(L.ax[2].deg[l[2]]) * double(nfree);
+      }
+    }
+    // MPO stages: carry (a', [s' done...], w_cur, [s remaining...], b)
+    // represent as Coords: [a', t_0..t_{k-1}, w, s_k..s_{n-1}, b]
+    std::set<Coords> cur = t1;  // k = 0: [a', w, s0, (s1), b]
+    for (int k = 0; k < nphys; ++k) {
+      std::set<Coords> nxt;
+      for (const Coords &c : cur) {
+        const int w = c[1 + k], s = c[2 + k];
+        auto it = W_by_in[k].find({w, s});
+        if (it == W_by_in[k].end()) continue;
+        // free dims of the intermediate other than (w, s)
+        int64_t mfree = L.ax[0].deg[c[0]] * R.ax[2].deg[c[rk]];   // a' and b
+        for (int j = 0; j < k; ++j) mfree *= p.W[j].ax[2].deg[c[1 + j]];
+        for (int j = k + 1; j < nphys; ++j) mfree *= psi.ax[1 + j].deg[c[2 + j]];
+        const int64_t kdim = p.W[k].ax[0].deg[w] * p.W[k].ax[1].deg[s];
+        for (int64_t wb : it->second) {
+          const Coords &wc = p.W[k].blk[wb];
+          Coords t = c;
+          t[1 + k] = wc[2];   // s_out
+          t[2 + k] = wc[3];   // w right
+          nxt.insert(t);
+          f += fl * double(mfree) * double(kdim) * double(p.W[k].ax[2].deg[wc[2]] * p.W[k].ax[3].deg[wc[3]]);
+        }
+      }
+      cur.swap(nxt);
+    }
+    // cur: [a', t..., z, b] ; contract (z, b) with R
+    for (const Coords &c : cur) {
+      const int z = c[1 + nphys], b = c[2 + nphys];
+      auto it = R_by_zb.find({z, b});
+      if (it == R_by_zb.end()) continue;
+      int64_t m = L.ax[0].deg[c[0]];
+      for (int j = 0; j < nphys; ++j) m *= p.W[j].ax[2].deg[c[1 + j]];
+      const int64_t kd = R.ax[1].deg[z] * R.ax[2].deg[b];
+      for (int64_t rb : it->second) {
+        Coords o{};
+        o[0] = c[0];
+        for (int j = 0; j < nphys; ++j) o[1 + j] = c[1 + j];
+        o[bax] = R.blk[rb][0];
+        out.insert(o);
+        f += fl * double(m) * double(kd) * double(R.ax[0].deg[R.blk[rb][0]]);
+      }
+    }
+    if (flops) *flops = f;
+    return out;
+  };
+
+  std::set<Coords> vset(psi.blk.begin(), psi.blk.end());
+  for (int iter = 0; iter < 16; ++iter) {
+    std::set<Coords> out = symbolic(vset, nullptr);
+    size_t before = vset.size();
+    vset.insert(out.begin(), out.end());
+    if (vset.size() == before) break;
+  }
+  symbolic(vset, &p.algo_flops);
+
+  // ---- vector layout ----------------------------------------------------------
+  p.vec.rank = rk;
+  p.vec.ax = psi.ax;
+  p.vec.blk.assign(vset.begin(), vset.end());   // std::set order = lexicographic
+  p.vec.off.resize(p.vec.blk.size());
+  {
+    int64_t o = 0;
+    for (size_t b = 0; b < p.vec.blk.size(); ++b) {
+      p.vec.off[b] = o;
+      p.vec.find[p.vec.blk[b]] = b;
+      o += p.vec.blk_elems(b);
+    }
+    p.vec.size = o;
+  }
+  std::map<int, std::vector<int64_t>> V_by_left;
+  for (size_t b = 0; b < p.vec.blk.size(); ++b) V_by_left[p.vec.blk[b][0]].push_back(b);
+
+
+  // ---- fused Krylov layout -----------------------------------------------------
+  struct Fused {
+    int64_t base = 0, N = 0;
+  };
+  std::map<int, Fused> fz;
+  std::vector<int64_t> fcol(p.vec.blk.size(), 0);
+  {
+    int64_t base = 0;
+    for (auto &kv : V_by_left) {
+      const int a = kv.first;
+      const int64_t Da = psi.ax[0].deg[a];
+      int64_t N = 0;
+      for (int64_t vb : kv.second) {
+        fcol[vb] = N;
+        N += p.vec.blk_elems(vb) / Da;
+      }
+      fz[a] = Fused{base, N};
+      base += Da * N;
+    }
+    QLG_CHECK(base == p.vec.size, "fused layout size mismatch");
+    for (size_t vb = 0; vb < p.vec.blk.size(); ++vb) {
+      const int a = p.vec.blk[vb][0];
+      const int64_t Da = psi.ax[0].deg[a], nb = p.vec.blk_elems(vb) / Da;
+      p.pack_in.push_back(CopyDesc{p.vec.off[vb], fz[a].base + fcol[vb], nb, fz[a].N, (int32_t)Da, (int32_t)nb});
+      p.pack_out.push_back(CopyDesc{fz[a].base + fcol[vb], p.vec.off[vb], fz[a].N, nb, (int32_t)Da, (int32_t)nb});
+    }
+  }
+
+  // ---- L stacks (ket-major) and stage 1 --------------------------------------------
+  struct LStack {
+    int64_t base = 0, rows = 0;
+    std::map<int64_t, int64_t> roff;   // L block -> first row in the stack
+  };
+  std::map<int, LStack> lst;
+  std::vector<CopyDesc> lcopy;
+  int64_t lint_elems = 0;
+  for (auto &kv : L_by_ket) {
+    const int a = kv.first;
+    if (!fz.count(a)) continue;
+    const int64_t Da = L.ax[2].deg[a];
+    LStack s;
+    s.base = lint_elems;
+    for (int64_t lb : kv.second) {
+      const int64_t r = L.ax[0].deg[L.blk[lb][0]] * L.ax[1].deg[L.blk[lb][1]];
+      s.roff[lb] = s.rows;
+      lcopy.push_back(CopyDesc{L.off[lb], s.base + s.rows * Da, Da, Da, (int32_t)r, (int32_t)Da});
+      s.rows += r;
+    }
+    lint_elems += s.rows * Da;
+    lst[a] = s;
+  }
+  Stage &s1 = p.st[0], &s2 = p.st[1], &s3 = p.st[2];
+  std::map<int, int64_t> tbase;
+  {
+    int64_t t1 = 0;
+    for (auto &kv : lst) {
+      const int a = kv.first;
+      const LStack &s = kv.second;
+      const int64_t Da = L.ax[2].deg[a], N = fz[a].N;
+      tbase[a] = t1;
+      GemmSeg g{};
+      g.a_slot = SLOT_L;
+      g.a_off = s.base;
+      g.lda = Da;
+      g.b_slot = SLOT_IN;
+      g.b_off = fz[a].base;
+      g.ldb = N;
+      g.K = (int32_t)Da;
+      GemmTask t{};
+      t.c_slot = SLOT_X;
+      t.c_off = t1;
+      t.ldc = N;
+      t.M = (int32_t)s.rows;
+      t.N = (int32_t)N;
+      t.seg_begin = (int32_t)s1.segs.size();
+      t.seg_count = 1;
+      s1.segs.push_back(g);
+      s1.tasks.push_back(t);
+      s1.flops += fl * double(s.rows) * double(N) * double(Da);
+      t1 += s.rows * N;
+    }
+    p.x_elems = t1;
+  }
+
+  // ---- R panels (bra-major) ------------------------------------------------------------
+  struct RPanel {
+    int64_t base = 0, K = 0;
+    std::map<std::pair<int, int>, int64_t> col;   // (z sector, b sector) -> column offset
+  };
+  std::map<int, RPanel> rpn;
+  std::vector<CopyDesc> rcopy;
+  int64_t rint_elems = 0;
+  for (auto &kv : R_by_bra) {
+    const int bp = kv.first;
+    RPanel P;
+    P.base = rint_elems;
+    for (int64_t rb : kv.second) {
+      P.col[{R.blk[rb][1], R.blk[rb][2]}] = P.K;
+      P.K += R.ax[1].deg[R.blk[rb][1]] * R.ax[2].deg[R.blk[rb][2]];
+    }
+    const int64_t Dbp = R.ax[0].deg[bp];
+    for (int64_t rb : kv.second) {
+      const int64_t w = R.ax[1].deg[R.blk[rb][1]] * R.ax[2].deg[R.blk[rb][2]];
+      rcopy.push_back(CopyDesc{R.off[rb], P.base + P.col[{R.blk[rb][1], R.blk[rb][2]}], w, P.K, (int32_t)Dbp,
+                               (int32_t)w});
+    }
+    rint_elems += Dbp * P.K;
+    rpn[bp] = P;
+  }
+
+  // ---- Y panels (per bra sector) and stage 3 ---------------------------------------------
+  struct YPanel {
+    int64_t base = 0, rows = 0, K = 0;
+  };
+  std::map<int, YPanel> ypn;
+  std::map<std::array<int64_t, 3>, int64_t> yrow;   // (vec block, comp0, comp1) -> first row
+  {
+    std::map<int, std::vector<int64_t>> V_by_right;
+    for (size_t vb = 0; vb < p.vec.blk.size(); ++vb) V_by_right[p.vec.blk[vb][bax]].push_back(vb);
+    int64_t ybase = 0;
+    for (auto &kv : V_by_right) {
+      const int bp = kv.first;
+      auto pit = rpn.find(bp);
+      YPanel Y;
+      Y.base = ybase;
+      Y.K = pit == rpn.end() ? 0 : pit->second.K;
+      const int64_t Dbp = R.ax[0].deg[bp];
+      const int32_t c_row0 = (int32_t)s3.crows.size();
+      for (int64_t vb : kv.second) {
+        const Coords &o = p.vec.blk[vb];
+        const int ap = o[0];
+        const int64_t Dap = L.ax[0].deg[ap];
+        const int64_t d0 = p.W[0].ax[2].deg[o[1]], d1 = nphys == 2 ? p.W[1].ax[2].deg[o[2]] : 1;
+        for (int c0 = 0; c0 < d0; ++c0)
+          for (int c1 = 0; c1 < d1; ++c1) {
+            yrow[{vb, c0, c1}] = Y.rows;
+            const int64_t col0 = fz[ap].base + fcol[vb] + (c0 * d1 + c1) * Dbp;
+            for (int64_t r = 0; r < Dap; ++r) s3.crows.push_back(col0 + r * fz[ap].N);
+            Y.rows += Dap;
+          }
+      }
+      GemmTask t{};
+      t.c_slot = SLOT_OUT;
+      t.c_off = 0;
+      t.ldc = Dbp;
+      t.M = (int32_t)Y.rows;
+      t.N = (int32_t)Dbp;
+      t.c_row = c_row0;
+      t.c_rowtab = 1;
+      t.seg_begin = (int32_t)s3.segs.size();
+      t.seg_count = 0;
+      if (Y.K > 0) {
+        GemmSeg g{};
+        g.a_slot = SLOT_Y;
+        g.a_off = Y.base;
+        g.lda = Y.K;
+        g.b_slot = SLOT_R;
+        g.b_off = pit->second.base;
+        g.ldb = Y.K;
+        g.K = (int32_t)Y.K;
+        g.flags = SEG_TRANS_B;
+        s3.segs.push_back(g);
+        t.seg_count = 1;
+        s3.flops += fl * double(Y.rows) * double(Dbp) * double(Y.K);
+      }
+      s3.tasks.push_back(t);
+      ybase += Y.rows * Y.K;
+      ypn[bp] = Y;
+    }
+    p.y_elems = ybase;
+  }
+
+  // ---- stage 2: channels (a'-sector, b-sector) -------------------------------------------
+  std::vector<double> Wdn[2];
+  std::vector<int64_t> wdims[2];
+  for (int k = 0; k < nphys; ++k) {
+    Wdn[k] = dense_copy(p.W[k], ctx->stream, ed);
+    for (int i = 0; i < 4; ++i) wdims[k].push_back(p.W[k].ax[i].dim());
+  }
+  auto Widx = [&](int k, int64_t a, int64_t b, int64_t c, int64_t d) {
+    return ((a * wdims[k][1] + b) * wdims[k][2] + c) * wdims[k][3] + d;
+  };
+  const auto so_w0 = sector_offsets(W0.ax[0]);
+  std::vector<int64_t> so_s[2], so_t[2];
+  for (int k = 0; k < nphys; ++k) {
+    so_s[k] = sector_offsets(psi.ax[1 + k]);
+    so_t[k] = sector_offsets(p.W[k].ax[2]);
+  }
+  const auto so_z = sector_offsets(W1.ax[3]);
+  using Key7 = std::array<int32_t, 7>;   // (t0 sect, t0 comp, t1 sect, t1 comp, z sect, z comp, b' sect)
+  struct InRow {
+    RowDesc d;
+    int64_t wg, sg0, sg1;
+    int wsec, ssec0, ssec1;
+  };
+  struct Chan {
+    int ap, b;
+    std::vector<InRow> in;
+    std::vector<Key7> out;
+  };
+  std::map<std::pair<int, int>, int> cid;
+  std::vector<Chan> ch;
+  for (auto &kv : lst) {
+    const int a = kv.first;
+    const LStack &stk = kv.second;
+    const int64_t Na = fz[a].N;
+    for (auto &lr : stk.roff) {
+      const int64_t lb = lr.first;
+      const Coords &l = L.blk[lb];
+      const int64_t Wm = L.ax[1].deg[l[1]];
+      for (int64_t vb : V_by_left[a]) {
+        const Coords &v = p.vec.blk[vb];
+        const int b = v[bax];
+        const int64_t Db = psi.ax[bax].deg[b];
+        const int64_t d0 = psi.ax[1].deg[v[1]], d1 = nphys == 2 ? psi.ax[2].deg[v[2]] : 1;
+        auto key = std::make_pair(l[0], b);
+        auto it = cid.find(key);
+        int c;
+        if (it == cid.end()) {
+          c = (int)ch.size();
+          cid[key] = c;
+          ch.push_back(Chan{l[0], b, {}, {}});
+        } else {
+          c = it->second;
+        }
+        for (int64_t w = 0; w < Wm; ++w)
+          for (int c0 = 0; c0 < d0; ++c0)
+            for (int c1 = 0; c1 < d1; ++c1) {
+              InRow r;
+              r.d.off = tbase[a] + (lr.second + w) * Na + fcol[vb] + (c0 * d1 + c1) * Db;
+              r.d.rs = Wm * Na;
+              r.wg = so_w0[l[1]] + w;
+              r.sg0 = so_s[0][v[1]] + c0;
+              r.sg1 = nphys == 2 ? so_s[1][v[2]] + c1 : 0;
+              r.wsec = l[1];
+              r.ssec0 = v[1];
+              r.ssec1 = nphys == 2 ? v[2] : 0;
+              ch[c].in.push_back(r);
+            }
+      }
+    }
+  }
+  std::vector<double> Ch;
+  int64_t coff = 0;
+  for (Chan &c : ch) {
+    // structurally reachable outputs that feed an existing R block and output block
+    std::set<Key7> outs;
+    for (const InRow &r : c.in) {
+      std::vector<std::pair<std::array<int, 2>, int>> fr{{{-1, -1}, r.wsec}};
+      const int ss[2] = {r.ssec0, r.ssec1};
+      for (int k = 0; k < nphys; ++k) {
+        std::vector<std::pair<std::array<int, 2>, int>> nf;
+        for (auto &f : fr) {
+          auto it = W_by_in[k].find({f.second, ss[k]});
+          if (it == W_by_in[k].end()) continue;
+          for (int64_t wb : it->second) {
+            auto t = f.first;
+            t[k] = p.W[k].blk[wb][2];
+            nf.push_back({t, p.W[k].blk[wb][3]});
+          }
+        }
+        fr.swap(nf);
+      }
+      for (auto &f : fr) {
+        const int z = f.second;
+        auto rit = R_by_zb.find({z, c.b});
+        if (rit == R_by_zb.end()) continue;
+        for (int64_t rb : rit->second) {
+          const int bp = R.blk[rb][0];
+          Coords o{};
+          o[0] = c.ap;
+          o[1] = f.first[0];
+          if (nphys == 2) o[2] = f.first[1];
+          o[bax] = bp;
+          if (!p.vec.find.count(o)) continue;
+          const int64_t dt0 = p.W[0].ax[2].deg[f.first[0]];
+          const int64_t dt1 = nphys == 2 ? p.W[1].ax[2].deg[f.first[1]] : 1;
+          const int64_t dz = W1.ax[3].deg[z];
+          for (int a0 = 0; a0 < dt0; ++a0)
+            for (int a1 = 0; a1 < dt1; ++a1)
+              for (int zz = 0; zz < dz; ++zz)
+                outs.insert(Key7{f.first[0], a0, nphys == 2 ? f.first[1] : 0, a1, z, zz, bp});
+        }
+      }
+    }
+    c.out.assign(outs.begin(), outs.end());
+    if (c.out.empty() || c.in.empty()) continue;
+    if ((int64_t)c.in.size() > kMaxChanIn)
+      throw Error(QLG_ERR_UNSUPPORTED, "a charge channel has more than " + std::to_string(kMaxChanIn) +
+                                           " MPO/physical input components");
+    const int64_t Dap = L.ax[0].deg[c.ap], Db = psi.ax[bax].deg[c.b];
+    ChanTask t{};
+    t.c_off = coff;
+    t.kin = (int32_t)c.in.size();
+    t.kout = (int32_t)c.out.size();
+    t.P = (int32_t)(Dap * Db);
+    t.Db = (int32_t)Db;
+    t.in_rows = (int32_t)s2.rows.size();
+    for (const InRow &r : c.in) s2.rows.push_back(r.d);
+    t.out_rows = (int32_t)s2.rows.size();
+    for (const Key7 &o : c.out) {
+      const int bp = o[6];
+      Coords ob{};
+      ob[0] = c.ap;
+      ob[1] = o[0];
+      if (nphys == 2) ob[2] = o[2];
+      ob[bax] = bp;
+      const int64_t vb = p.vec.find[ob];
+      const YPanel &Y = ypn[bp];
+      const int64_t row0 = yrow[{vb, o[1], o[3]}];
+      RowDesc d;
+      d.off = Y.base + row0 * Y.K + rpn[bp].col[{o[4], c.b}] + (int64_t)o[5] * Db;
+      d.rs = Y.K;
+      s2.rows.push_back(d);
+    }
+    // combined operator rows (outputs) x columns (inputs)
+    const int64_t K = (int64_t)c.in.size();
+    Ch.resize((coff + (int64_t)c.out.size() * K) * ed, 0.0);
+    for (size_t r = 0; r < c.out.size(); ++r) {
+      const Key7 &o = c.out[r];
+      const int64_t tg0 = so_t[0][o[0]] + o[1];
+      const int64_t tg1 = nphys == 2 ? so_t[1][o[2]] + o[3] : 0;
+      const int64_t zg = so_z[o[4]] + o[5];
+      for (int64_t q = 0; q < K; ++q) {
+        const InRow &in = c.in[q];
+        double vr = 0, vi = 0;
+        if (nphys == 1) {
+          const int64_t i = Widx(0, in.wg, in.sg0, tg0, zg);
+          vr = Wdn[0][i * ed];
+          if (ed == 2) vi = Wdn[0][i * ed + 1];
+        } else {
+          for (int64_t y = 0; y < wdims[0][3]; ++y) {
+            const int64_t i0 = Widx(0, in.wg, in.sg0, tg0, y), i1 = Widx(1, y, in.sg1, tg1, zg);
+            if (ed == 1) {
+              vr += Wdn[0][i0] * Wdn[1][i1];
+            } else {
+              const double ar = Wdn[0][2 * i0], ai = Wdn[0][2 * i0 + 1];
+              const double br = Wdn[1][2 * i1], bi = Wdn[1][2 * i1 + 1];
+              vr += ar * br - ai * bi;
+              vi += ar * bi + ai * br;
+            }
+          }
+        }
+        const int64_t e = coff + (int64_t)r * K + q;
+        Ch[e * ed] = vr;
+        if (ed == 2) Ch[e * ed + 1] = vi;
+      }
+    }
+    coff += (int64_t)c.out.size() * K;
+    s2.chan.push_back(t);
+    s2.flops += fl * double(t.kout) * double(t.kin) * double(t.P);
+  }
+  p.c_elems = coff;
+
+  // ---- bounds of every task, then tiles and uploads --------------------------------
+  {
+    int64_t slot_elems[NSLOTS] = {0};
+    slot_elems[SLOT_IN] = slot_elems[SLOT_OUT] = p.vec.size;
+    slot_elems[SLOT_L] = lint_elems;
+    slot_elems[SLOT_R] = rint_elems;
+    slot_elems[SLOT_X] = p.x_elems;
+    slot_elems[SLOT_Y] = p.y_elems;
+    slot_elems[SLOT_C] = p.c_elems;
+    for (int i = 0; i < 3; ++i) validate_stage(p.st[i], slot_elems);
+    auto check_copies = [](const std::vector<CopyDesc> &v, int64_t src_n, int64_t dst_n, const char *what) {
+      for (const CopyDesc &c : v)
+        QLG_CHECK(c.rows > 0 && c.cols > 0 && c.src >= 0 && c.dst >= 0 && c.sld >= c.cols && c.dld >= c.cols &&
+                      c.src + (int64_t)(c.rows - 1) * c.sld + c.cols <= src_n &&
+                      c.dst + (int64_t)(c.rows - 1) * c.dld + c.cols <= dst_n,
+                  std::string("copy list out of bounds: ") + what);
+    };
+    auto extent = [](const Ten &t) {
+      int64_t e = 0;
+      for (size_t b = 0; b < t.blk.size(); ++b) e = std::max(e, t.off[b] + t.blk_elems(b));
+      return e;
+    };
+    check_copies(p.pack_in, p.vec.size, p.vec.size, "pack");
+    check_copies(p.pack_out, p.vec.size, p.vec.size, "unpack");
+    check_copies(lcopy, extent(L), lint_elems, "L repack");
+    check_copies(rcopy, extent(R), rint_elems, "R repack");
+  }
+  const char *pin = std::getenv("QLG_TILES");
+  int pinned[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  if (pin) std::sscanf(pin, "%dx%d,%dx%d,%dx%d", &pinned[0][0], &pinned[0][1], &pinned[1][0], &pinned[1][1],
+                       &pinned[2][0], &pinned[2][1]);
+  for (int i = 0; i < 3; ++i) {
+    if (pinned[i][0] > 0) {
+      p.st[i].make_tiles(pinned[i][0], pinned[i][1]);
+    } else if (i == 1) {
+      // MPO stage: HBM-bound; cover each channel's outputs with one or two row tiles
+      double c32 = 0, c64 = 0;
+      for (const ChanTask &c : s2.chan) {
+        c32 += double((c.kout + 31) / 32 * 32) * c.kin * c.P + 2.0e4 * ((c.kout + 31) / 32);
+        c64 += double((c.kout + 63) / 64 * 64) * c.kin * c.P + 2.0e4 * ((c.kout + 63) / 64);
+      }
+      p.st[i].make_tiles(c64 <= c32 ? 64 : 32, 64);
+    } else {
+      autotune_tiles(p.st[i], p.dtype == QLG_C128);
+    }
+    p.st[i].upload(ctx->stream);
+  }
+  p.d_pack_in.ensure(std::max<size_t>(1, p.pack_in.size()) * sizeof(CopyDesc));
+  p.d_pack_out.ensure(std::max<size_t>(1, p.pack_out.size()) * sizeof(CopyDesc));
+  if (!p.pack_in.empty()) {
+    QLG_HIP(hipMemcpyAsync(p.d_pack_in.p, p.pack_in.data(), p.pack_in.size() * sizeof(CopyDesc),
+                           hipMemcpyHostToDevice, ctx->stream));
+    QLG_HIP(hipMemcpyAsync(p.d_pack_out.p, p.pack_out.data(), p.pack_out.size() * sizeof(CopyDesc),
+                           hipMemcpyHostToDevice, ctx->stream));
+  }
+  const size_t eb = ed * sizeof(double);
+  p.X.ensure(std::max<int64_t>(p.x_elems, 1) * eb);
+  p.Y.ensure(std::max<int64_t>(p.y_elems, 1) * eb);
+  p.C.ensure(std::max<int64_t>(p.c_elems, 1) * eb);
+  p.Lint.ensure(std::max<int64_t>(lint_elems, 1) * eb);
+  p.Rint.ensure(std::max<int64_t>(rint_elems, 1) * eb);
+  // Y entries no channel writes stay zero for the lifetime of the plan
+  QLG_HIP(hipMemsetAsync(p.Y.p, 0, std::max<int64_t>(p.y_elems, 1) * eb, ctx->stream));
+  if (p.c_elems > 0)
+    QLG_HIP(hipMemcpyAsync(p.C.p, Ch.data(), p.c_elems * eb, hipMemcpyHostToDevice, ctx->stream));
+  {
+    DevBuf dl, dr;
+    dl.ensure(std::max<size_t>(1, lcopy.size()) * sizeof(CopyDesc));
+    dr.ensure(std::max<size_t>(1, rcopy.size()) * sizeof(CopyDesc));
+    if (!lcopy.empty())
+      QLG_HIP(hipMemcpyAsync(dl.p, lcopy.data(), lcopy.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, ctx->stream));
+    if (!rcopy.empty())
+      QLG_HIP(hipMemcpyAsync(dr.p, rcopy.data(), rcopy.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, ctx->stream));
+    launch_block_copy(ed == 2, dl.as<CopyDesc>(), (int)lcopy.size(), static_cast<const double *>(L.data),
+                      p.Lint.as<double>(), ctx->stream);
+    launch_block_copy(ed == 2, dr.as<CopyDesc>(), (int)rcopy.size(), static_cast<const double *>(R.data),
+                      p.Rint.as<double>(), ctx->stream);
+    QLG_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (const char *v = std::getenv("QLG_VERBOSE"); v && v[0] == '1') {
+    const char *names[3] = {"L.psi", "MPO", ".R^T"};
+    for (int i = 0; i < 3; ++i) {
+      const Stage &s = p.st[i];
+      std::fprintf(stderr, "[qlg plan] stage %d (%s): tasks %zu tiles %zu tile %dx%d gflop %.3f\n", i + 1, names[i],
+                   i == 1 ? s.chan.size() : s.tasks.size(), s.tiles.size(), s.bm, s.bn, s.flops / 1e9);
+    }
+    std::fprintf(stderr, "[qlg plan] vector %lld, T1 %lld, Y %lld, C %lld, L stacks %lld, R panels %lld, algo gflop %.3f\n",
+                 (long long)p.vec.size, (long long)p.x_elems, (long long)p.y_elems, (long long)p.c_elems,
+                 (long long)lint_elems, (long long)rint_elems, p.algo_flops / 1e9);
+  }
+  return P.release();
+}
+
+// ---------------------------------------------------------------------------
+Plan::~Plan() {
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void Plan::to_fused(const double *canon, double *fused) {
+  launch_block_copy(dtype == QLG_C128, d_pack_in.as<CopyDesc>(), (int)pack_in.size(), canon, fused, ctx->stream);
+}
+
+void Plan::to_canonical(const double *fused, double *canon) {
+  launch_block_copy(dtype == QLG_C128, d_pack_out.as<CopyDesc>(), (int)pack_out.size(), fused, canon, ctx->stream);
+}
+
+void Plan::matvec_fused(const double *in, double *out) {
+  Slots s{};
+  s.p[SLOT_IN] = const_cast<double *>(in);
+  s.p[SLOT_OUT] = out;
+  s.p[SLOT_L] = Lint.as<double>();
+  s.p[SLOT_R] = Rint.as<double>();
+  s.p[SLOT_X] = X.as<double>();
+  s.p[SLOT_Y] = Y.as<double>();
+  s.p[SLOT_C] = C.as<double>();
+  s.crows = st[2].d_crows.as<int64_t>();
+  hipStream_t stream = ctx->stream;
+  const bool cplx = dtype == QLG_C128;
+  if (timing) {
+    for (auto &e : ev)
+      if (!e) QLG_HIP(hipEventCreate(&e));
+    QLG_HIP(hipEventRecord(ev[0], stream));
+  }
+  Stage &a = st[0];
+  launch_grouped_gemm_f64(a.bm, a.bn, cplx, a.d_tiles.as<GemmTile>(), a.d_tasks.as<GemmTask>(),
+                          a.d_segs.as<GemmSeg>(), s, (int)a.tiles.size(), stream);
+  if (timing) QLG_HIP(hipEventRecord(ev[1], stream));
+  Stage &b = st[1];
+  launch_mpo_stage(b.bm, b.bn, cplx, b.d_tiles.as<GemmTile>(), b.d_chan.as<ChanTask>(), b.d_rows.as<RowDesc>(),
+                   C.as<double>(), X.as<double>(), Y.as<double>(), (int)b.tiles.size(), stream);
+  if (timing) QLG_HIP(hipEventRecord(ev[2], stream));
+  Stage &c = st[2];
+  launch_grouped_gemm_f64(c.bm, c.bn, cplx, c.d_tiles.as<GemmTile>(), c.d_tasks.as<GemmTask>(),
+                          c.d_segs.as<GemmSeg>(), s, (int)c.tiles.size(), stream);
+  if (timing) QLG_HIP(hipEventRecord(ev[3], stream));
+}
+
+void Plan::matvec(const double *in, double *out) {
+  const size_t bytes = std::max<int64_t>(vec.size, 1) * elem_doubles(dtype) * sizeof(double);
+  vtmp_in.ensure(bytes);
+  vtmp_out.ensure(bytes);
+  to_fused(in, vtmp_in.as<double>());
+  matvec_fused(vtmp_in.as<double>(), vtmp_out.as<double>());
+  to_canonical(vtmp_out.as<double>(), out);
+}
+
+}  // namespace qlg

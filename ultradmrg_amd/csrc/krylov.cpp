@@ -54,9 +54,11 @@ struct Workspace {
     QLG_HIP(hipStreamSynchronize(c.stream));
     for (int i = 0; i < nvals; ++i) host[i] = c.host_scalars[i];
   }
-  // y <- sum_i coef_i * slot(i) for i < k (real or complex coefficients)
-  void combine(int64_t k, const double *coefs, bool cplx_coef, double *y) {
+  // y (canonical layout) <- sum_i coef_i * slot(i) for i < k (real or complex coefficients)
+  void combine(int64_t k, const double *coefs, bool cplx_coef, double *y_canonical) {
     Ctx &c = *p.ctx;
+    p.vtmp_out.ensure((size_t)nd * sizeof(double));
+    double *y = p.vtmp_out.as<double>();
     std::vector<const double *> ptrs(k);
     for (int64_t i = 0; i < k; ++i) ptrs[i] = slot(i);
     QLG_HIP(hipMemcpyAsync(p.coef_buf.p, coefs, k * (cplx_coef ? 2 : 1) * sizeof(double), hipMemcpyHostToDevice,
@@ -64,6 +66,7 @@ struct Workspace {
     QLG_HIP(hipMemcpyAsync(p.ptr_buf.p, ptrs.data(), k * sizeof(double *), hipMemcpyHostToDevice, c.stream));
     vec_lincomb(c, p.dtype, n, (int)k, p.coef_buf.as<double>(), p.ptr_buf.as<const double *>(), cplx_coef, 0.0, 0.0,
                 false, y);
+    p.to_canonical(y, y_canonical);
     // the host arrays above must stay alive until the copies are done
     QLG_HIP(hipStreamSynchronize(c.stream));
   }
@@ -81,13 +84,13 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
   double h[2];
 
   // v0 = psi / ||psi||   (QuasiNormalize, lanczos_vmps_solver_impl.h:141)
-  QLG_HIP(hipMemcpyAsync(w.slot(0), psi, w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  p.to_fused(psi, w.slot(0));  // the Krylov basis uses the fused layout
   vec_nrm2(c, p.dtype, w.n, w.slot(0), w.dev_tmp);
   w.fetch(w.dev_tmp, h, 1);
   QLG_CHECK(h[0] > 0.0, "initial state has zero norm");
   vec_scal(c, p.dtype, w.n, 1.0 / h[0], 0.0, w.slot(0));
   // a_0 and the first residual
-  p.matvec(w.slot(0), w.slot(1));
+  p.matvec_fused(w.slot(0), w.slot(1));
   vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0);
   vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0, nullptr, nullptr, w.dev_b + 0);
   QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -102,7 +105,7 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
     // slot m holds gamma_m (un-normalised residual), norm_gamma = ||gamma_m||
     if (norm_gamma == 0.0) {
       if (m == 1) {
-        QLG_HIP(hipMemcpyAsync(psi, w.slot(0), w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        p.to_canonical(w.slot(0), psi);
         QLG_HIP(hipStreamSynchronize(c.stream));
         res.iters = m;
         res.gs_eng = energy0;
@@ -120,7 +123,7 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
     N[m] = norm_gamma * norm_gamma;
     b[m - 1] = norm_gamma;
     vec_scal(c, p.dtype, w.n, 1.0 / norm_gamma, 0.0, w.slot(m));
-    p.matvec(w.slot(m), w.slot(m + 1));
+    p.matvec_fused(w.slot(m), w.slot(m + 1));
     vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m);
     // speculative residual for iteration m+1: H v_m - a_m v_m - b_{m-1} v_{m-1}
     vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m, w.slot(m - 1), w.dev_b + (m - 1),
@@ -158,13 +161,13 @@ void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &p
   double h[2];
 
   // Normalize keeps the initial norm (lanczos_expmv_solver_impl.h:93)
-  QLG_HIP(hipMemcpyAsync(w.slot(0), psi, w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  p.to_fused(psi, w.slot(0));  // the Krylov basis uses the fused layout
   vec_nrm2(c, p.dtype, w.n, w.slot(0), w.dev_tmp);
   w.fetch(w.dev_tmp, h, 1);
   const double initial_norm = h[0];
   QLG_CHECK(initial_norm > 0.0, "initial state has zero norm");
   vec_scal(c, p.dtype, w.n, 1.0 / initial_norm, 0.0, w.slot(0));
-  p.matvec(w.slot(0), w.slot(1));
+  p.matvec_fused(w.slot(0), w.slot(1));
   // a_m = Re <H v_m | v_m>; the complex dot writes (re, im)
   vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_tmp + 2);
   QLG_HIP(hipMemcpyAsync(w.dev_a + 0, w.dev_tmp + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -182,7 +185,7 @@ void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &p
       if (m == 1) {
         // psi <- ||psi0|| exp(-i tau a_0) v_0
         const double ph = -tau * a[0];
-        QLG_HIP(hipMemcpyAsync(psi, w.slot(0), w.nd * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        p.to_canonical(w.slot(0), psi);
         vec_scal(c, p.dtype, w.n, initial_norm * std::cos(ph), initial_norm * std::sin(ph), psi);
         QLG_HIP(hipStreamSynchronize(c.stream));
         return;
@@ -195,7 +198,7 @@ void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &p
     }
     b[m - 1] = norm_gamma;
     vec_scal(c, p.dtype, w.n, 1.0 / norm_gamma, 0.0, w.slot(m));
-    p.matvec(w.slot(m), w.slot(m + 1));
+    p.matvec_fused(w.slot(m), w.slot(m + 1));
     vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_tmp + 2);
     QLG_HIP(hipMemcpyAsync(w.dev_a + m, w.dev_tmp + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m, w.slot(m - 1), w.dev_b + (m - 1),

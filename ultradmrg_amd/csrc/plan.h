@@ -23,7 +23,10 @@ struct Stage {
   std::vector<GemmTile> tiles;
   std::vector<GemmTask> tasks;
   std::vector<GemmSeg> segs;
-  DevBuf d_tiles, d_tasks, d_segs;
+  std::vector<int64_t> crows;      // output-row address table (tasks with c_rowtab)
+  std::vector<ChanTask> chan;      // MPO-stage channels (instead of tasks)
+  std::vector<RowDesc> rows;       // their gathered / scattered rows
+  DevBuf d_tiles, d_tasks, d_segs, d_crows, d_chan, d_rows;
   int bm = 64, bn = 64;
   double flops = 0;      // useful 2*M*N*K issued by this stage
   void upload(hipStream_t st);
@@ -65,8 +68,19 @@ struct Plan {
   // timing
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // Inside the library the Krylov vectors use the fused layout: for each left
+  // sector a of the state, one row-major matrix [D_a x N_a] whose column
+  // ranges are the state's blocks (a, ...).  pack_in / pack_out convert
+  // between it and the canonical layout of the boundary.
+  std::vector<CopyDesc> pack_in, pack_out;
+  DevBuf d_pack_in, d_pack_out;
+  DevBuf Lint, Rint;            // environments repacked (ket-major L stacks, bra-major R panels)
+  DevBuf vtmp_in, vtmp_out;     // staging for qlg_matvec on canonical vectors
   ~Plan();
-  void matvec(const double *in, double *out);
+  void matvec(const double *in, double *out);        // canonical in / out
+  void matvec_fused(const double *in, double *out);  // fused in / out
+  void to_fused(const double *canon, double *fused);
+  void to_canonical(const double *fused, double *canon);
   void execute(const void *in0, const void *in1, void *out);
 };
 
@@ -76,6 +90,12 @@ Plan *build_tensordot_plan(Ctx *ctx, const qlg_bsten *A, const qlg_bsten *B, int
 Plan *build_env_plan(Ctx *ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side);
 
 void check_pair(const Axis &a, const Axis &b, const char *what);
+std::vector<int64_t> sector_offsets(const Axis &a);
+std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed);
+void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
+void launch_mpo_stage(int bm, int bn, bool cplx, const GemmTile *tiles, const ChanTask *tasks, const RowDesc *rows,
+                      const double *C, const double *X, double *Y, int ntiles, hipStream_t st);
+void launch_block_copy(bool cplx, const CopyDesc *d, int n, const double *src, double *dst, hipStream_t st);
 void validate_stage(const Stage &s, const int64_t *slot_elems);
 void autotune_tiles(Stage &s, bool cplx);
 

@@ -1,0 +1,35 @@
+// Device helpers shared by the tiled MFMA kernels: raw buffer loads with
+// hardware range checking.  A buffer resource spans 2^31 bytes from an
+// operand's base address; an element that lies outside the tile's logical
+// operand is given the byte offset kOob, which the buffer unit answers with
+// zeros.  The loads therefore need neither a branch nor a select on the
+// loaded value, so the compiler can keep them in flight across the MFMA
+// work of the previous K slice.  Every in-range operand offset is checked on
+// the host to stay below 2^31 bytes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qlg {
+
+constexpr uint32_t kOob = 0x80000000u;
+constexpr int kRsrcWord3 = 0x00020000;  // raw (untyped) buffer, gfx9 family
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)kOob, kRsrcWord3);
+}
+
+__device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+
+__device__ inline double2 load_c128(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const u2 lo = {v.x, v.y}, hi = {v.z, v.w};
+  return double2{__builtin_bit_cast(double, lo), __builtin_bit_cast(double, hi)};
+}
+
+}  // namespace qlg
