@@ -60,8 +60,6 @@ struct ChanTask {
   int32_t in_rows, out_rows;
 };
 
-// input rows of one channel are staged in LDS by the MPO-stage kernel
-constexpr int kMaxChanIn = 1024;
 
 struct RowDesc {
   int64_t off;   // element offset of position (0, 0)

@@ -460,9 +460,6 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     }
     c.out.assign(outs.begin(), outs.end());
     if (c.out.empty() || c.in.empty()) continue;
-    if ((int64_t)c.in.size() > kMaxChanIn)
-      throw Error(QLG_ERR_UNSUPPORTED, "a charge channel has more than " + std::to_string(kMaxChanIn) +
-                                           " MPO/physical input components");
     const int64_t Dap = L.ax[0].deg[c.ap], Db = psi.ax[bax].deg[c.b];
     ChanTask t{};
     t.c_off = coff;
@@ -537,6 +534,10 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     slot_elems[SLOT_Y] = p.y_elems;
     slot_elems[SLOT_C] = p.c_elems;
     for (int i = 0; i < 3; ++i) validate_stage(p.st[i], slot_elems);
+    // the MPO stage gathers from the whole first intermediate through 32-bit
+    // byte offsets, and reads each channel operator the same way
+    if (p.x_elems * ed * 8 >= (int64_t(1) << 31))
+      throw Error(QLG_ERR_UNSUPPORTED, "first H_eff intermediate exceeds 2^31 bytes");
     auto check_copies = [](const std::vector<CopyDesc> &v, int64_t src_n, int64_t dst_n, const char *what) {
       for (const CopyDesc &c : v)
         QLG_CHECK(c.rows > 0 && c.cols > 0 && c.src >= 0 && c.dst >= 0 && c.sld >= c.cols && c.dld >= c.cols &&
@@ -562,13 +563,23 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     if (pinned[i][0] > 0) {
       p.st[i].make_tiles(pinned[i][0], pinned[i][1]);
     } else if (i == 1) {
-      // MPO stage: HBM-bound; cover each channel's outputs with one or two row tiles
-      double c32 = 0, c64 = 0;
-      for (const ChanTask &c : s2.chan) {
-        c32 += double((c.kout + 31) / 32 * 32) * c.kin * c.P + 2.0e4 * ((c.kout + 31) / 32);
-        c64 += double((c.kout + 63) / 64 * 64) * c.kin * c.P + 2.0e4 * ((c.kout + 63) / 64);
+      // MPO stage: HBM-bound.  X is re-read once per row tile, so the row
+      // tile covers K_out where the MFMA padding stays cheap; the cost counts
+      // X reads per row tile plus padded MFMA work at ~10 flops per byte
+      double best = 1e300;
+      int bm_best = 32;
+      for (int bm : {16, 32, 64}) {
+        double cost = 0;
+        for (const ChanTask &c : s2.chan) {
+          const double rt = (c.kout + bm - 1) / bm;
+          cost += rt * c.kin * double(c.P) * 8.0 + rt * bm * double(c.kin) * c.P * 2.0 / 10.0;
+        }
+        if (cost < best) {
+          best = cost;
+          bm_best = bm;
+        }
       }
-      p.st[i].make_tiles(c64 <= c32 ? 64 : 32, 64);
+      p.st[i].make_tiles(bm_best, bm_best == 64 ? 64 : 128);
     } else {
       autotune_tiles(p.st[i], p.dtype == QLG_C128);
     }
@@ -616,6 +627,23 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     std::fprintf(stderr, "[qlg plan] vector %lld, T1 %lld, Y %lld, C %lld, L stacks %lld, R panels %lld, algo gflop %.3f\n",
                  (long long)p.vec.size, (long long)p.x_elems, (long long)p.y_elems, (long long)p.c_elems,
                  (long long)lint_elems, (long long)rint_elems, p.algo_flops / 1e9);
+    int kin_max = 0, kout_max = 0, p_max = 0, nbig = 0;
+    double xreads = 0, kin_avg = 0, kout_avg = 0, wsum = 0;
+    for (const ChanTask &c : s2.chan) {
+      kin_max = std::max(kin_max, c.kin);
+      kout_max = std::max(kout_max, c.kout);
+      p_max = std::max(p_max, c.P);
+      nbig += c.kin > 256;
+      xreads += double((c.kout + s2.bm - 1) / s2.bm) * c.kin * c.P;
+      kin_avg += double(c.kin) * c.P;
+      kout_avg += double(c.kout) * c.P;
+      wsum += c.P;
+    }
+    std::fprintf(stderr,
+                 "[qlg plan] MPO channels: kin max %d avg %.1f, kout max %d avg %.1f, P max %d avg %.0f, "
+                 "kin>256: %d, X element reads %.0f (%.2fx T1)\n",
+                 kin_max, kin_avg / std::max(wsum, 1.0), kout_max, kout_avg / std::max(wsum, 1.0), p_max,
+                 wsum / std::max<size_t>(s2.chan.size(), 1), nbig, xreads, xreads / std::max<double>(p.x_elems, 1));
   }
   return P.release();
 }

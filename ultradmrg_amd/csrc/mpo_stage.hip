@@ -3,22 +3,31 @@
 // operator of the channel (row-major [K_out x K_in]) and X / Y are sets of
 // rows whose elements live in 2-D strided panels of the stage-1 result and
 // of the stage-3 operand: position n of a row is (a' = n / Db, b = n % Db)
-// at  off + a' * rs + b.  The kernel is the grouped FP64 MFMA GEMM of
-// gemm_f64.hip with the B rows gathered and the C rows scattered through
-// per-row (off, rs) tables; it reads every X element once and writes every
-// Y element once (HBM-bound).
+// at  off + a' * rs + b.
+//
+// The stage is bound by HBM traffic: every X element is read once per row
+// tile (BM >= K_out in the common case, so once) and every Y element is
+// written once, and K_in is short (tens).  A workgroup therefore keeps its
+// memory traffic in flight in one burst: the four waves split the tile's
+// columns, the C chunk (BM x KC) is staged in LDS, and each lane loads the
+// X values of its MFMA B fragments for the whole K chunk straight into
+// registers before the first MFMA.  Raw buffer loads return zero outside
+// the channel (tile_io.h).  The first kRowWin input-row descriptors of a
+// channel are staged in LDS; further ones (channels with many MPO/physical
+// components) are read from memory.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
 #include "gemm.h"
+#include "tile_io.h"
 
 namespace qlg {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 namespace {
-constexpr int BK = 16;
-constexpr int PAD = 1;
+constexpr int KC = 32;        // K chunk held in registers per round
+constexpr int kRowWin = 256;
 }  // namespace
 
 template <int BM, int BN, bool CPLX>
@@ -27,36 +36,39 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
                                                         const RowDesc *__restrict__ rows, const double *__restrict__ C,
                                                         const double *__restrict__ X, double *__restrict__ Y,
                                                         int ntiles) {
-  constexpr int WM = BM / 2, WN = BN / 2;
-  constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int A_PER = BM * BK / 256;
-  constexpr int B_PER = BK * BN / 256;
+  constexpr int WN = BN / 4;                // columns per wave
+  constexpr int FM = BM / 16, FN = WN / 16;
   constexpr int NP = CPLX ? 2 : 1;
-  static_assert(A_PER >= 1 && B_PER >= 1, "tile too small");
-  static_assert(256 % BN == 0, "a thread keeps one column across its B loads");
-  __shared__ double As[2][NP][BK][BM + PAD];
-  __shared__ double Bs[2][NP][BK][BN + PAD];
-  __shared__ RowDesc rin[kMaxChanIn];
+  constexpr uint32_t EB = CPLX ? 16u : 8u;
+  constexpr int KS = KC / 4;                // MFMA k-steps per chunk
+  static_assert(FM >= 1 && FN >= 1, "tile too small");
+  __shared__ double As[NP][KC][BM + 1];
+  __shared__ RowDesc rin[kRowWin];
   __shared__ RowDesc rout[BM];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   if ((int)blockIdx.x >= ntiles) return;
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
   const ChanTask tk = tasks[tl.task];
   const int M = tk.kout, N = tk.P, K = tk.kin;
   const double *Cm = C + tk.c_off * NP;
-  // row descriptors of this channel's inputs and of the tile's outputs
-  for (int i = tid; i < K; i += 256) rin[i] = rows[tk.in_rows + i];
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(X);
+  const RowDesc *gin = rows + tk.in_rows;
+  for (int i = tid; i < min(K, kRowWin); i += 256) rin[i] = gin[i];
   for (int i = tid; i < BM; i += 256) rout[i] = rows[tk.out_rows + min(tl.m0 + i, M - 1)];
-  __syncthreads();
 
-  // this thread's fixed column for B loads
-  const int gn_b = tl.n0 + tid % BN;
-  const bool col_ok = gn_b < N;
-  const int64_t ap_b = col_ok ? gn_b / tk.Db : 0, b_b = col_ok ? gn_b % tk.Db : 0;
+  // this lane's columns: (a', b) positions of its B fragments
+  int apj[FN], bj[FN];
+  bool okj[FN];
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int col = tl.n0 + wave * WN + j * 16 + (lane & 15);
+    okj[j] = col < N;
+    apj[j] = okj[j] ? col / tk.Db : 0;
+    bj[j] = okj[j] ? col % tk.Db : 0;
+  }
 
   dbl4 acc[NP][FM][FN];
 #pragma unroll
@@ -65,107 +77,75 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j) acc[p][i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-  double ra[NP][A_PER], rb[NP][B_PER];
 
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256;
-      const int m = e / BK, k = e % BK;
-      const int gm = tl.m0 + m, gk = k0 + k;
+  for (int kc = 0; kc < K; kc += KC) {
+    __syncthreads();  // descriptors staged / previous chunk of As consumed
+    for (int e = tid; e < BM * KC; e += 256) {
+      const int m = e / KC, k = e % KC;
+      const int gm = tl.m0 + m, gk = kc + k;
       const bool ok = gm < M && gk < K;
       const int64_t ix = ok ? (int64_t)gm * K + gk : 0;
       if constexpr (CPLX) {
         const double2 v = reinterpret_cast<const double2 *>(Cm)[ix];
-        ra[0][i] = ok ? v.x : 0.0;
-        ra[1][i] = ok ? v.y : 0.0;
+        As[0][k][m] = ok ? v.x : 0.0;
+        As[1][k][m] = ok ? v.y : 0.0;
       } else {
-        const double v = Cm[ix];
-        ra[0][i] = ok ? v : 0.0;
+        As[0][k][m] = ok ? Cm[ix] : 0.0;
       }
     }
+    // B fragments of the whole chunk: lane holds X[row kc + 4 s + lane / 16][its column j]
+    double rb[NP][KS][FN];
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int e = tid + i * 256;
-      const int gk = k0 + e / BN;
-      const bool ok = col_ok && gk < K;
-      const RowDesc rd = rin[ok ? gk : 0];
-      const int64_t ix = ok ? rd.off + ap_b * rd.rs + b_b : rin[0].off;
-      if constexpr (CPLX) {
-        const double2 v = reinterpret_cast<const double2 *>(X)[ix];
-        rb[0][i] = ok ? v.x : 0.0;
-        rb[1][i] = ok ? v.y : 0.0;
-      } else {
-        const double v = X[ix];
-        rb[0][i] = ok ? v : 0.0;
+    for (int st = 0; st < KS; ++st) {
+      const int gk = kc + 4 * st + (lane >> 4);
+      const bool kok = gk < K;
+      const RowDesc rd = K <= kRowWin ? rin[kok ? gk : 0] : gin[kok ? gk : 0];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const uint32_t off =
+            (kok && okj[j]) ? (uint32_t)((int)rd.off + apj[j] * (int)rd.rs + bj[j]) * EB : kOob;
+        if constexpr (CPLX) {
+          const double2 v = load_c128(xr, off);
+          rb[0][st][j] = v.x;
+          rb[1][st][j] = v.y;
+        } else {
+          rb[0][st][j] = load_f64(xr, off);
+        }
       }
     }
-  };
-  auto store = [&](int buf) {
+    __syncthreads();  // C chunk staged
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256;
+    for (int st = 0; st < KS; ++st) {
+      double af[NP][FM];
 #pragma unroll
-      for (int p = 0; p < NP; ++p) As[buf][p][e % BK][e / BK] = ra[p][i];
-    }
+      for (int p = 0; p < NP; ++p)
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int e = tid + i * 256;
-#pragma unroll
-      for (int p = 0; p < NP; ++p) Bs[buf][p][e / BN][e % BN] = rb[p][i];
-    }
-  };
-
-  int buf = 0;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    const bool more = k0 + BK < K;
-    if (more) load(k0 + BK);
-    const int krem = K - k0;
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      if (kk >= krem) break;
-      double af[NP][FM], bf[NP][FN];
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i) af[p][i] = As[buf][p][kk + (lane >> 4)][wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) bf[p][j] = Bs[buf][p][kk + (lane >> 4)][wn0 + j * 16 + (lane & 15)];
-      }
+        for (int i = 0; i < FM; ++i) af[p][i] = As[p][4 * st + (lane >> 4)][i * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], bf[0][j], acc[0][i][j], 0, 0, 0);
+          acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], rb[0][st][j], acc[0][i][j], 0, 0, 0);
           if constexpr (CPLX) {
-            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[1][i], bf[1][j], acc[0][i][j], 0, 0, 0);
-            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], bf[1][j], acc[1][i][j], 0, 0, 0);
-            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[1][i], bf[0][j], acc[1][i][j], 0, 0, 0);
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[1][i], rb[1][st][j], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], rb[1][st][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[1][i], rb[0][st][j], acc[1][i][j], 0, 0, 0);
           }
         }
     }
-    if (!more) break;
-    store(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
   }
   // scatter: row m -> rout[m - m0], column n -> (n / Db, n % Db)
 #pragma unroll
   for (int j = 0; j < FN; ++j) {
-    const int col = tl.n0 + wn0 + j * 16 + (lane & 15);
-    if (col >= N) continue;
-    const int64_t ap = col / tk.Db, b = col % tk.Db;
+    if (!okj[j]) continue;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int lr = wm0 + i * 16 + (lane >> 4) + 4 * r;
+        const int lr = i * 16 + (lane >> 4) + 4 * r;
         if (tl.m0 + lr < M) {
           const RowDesc rd = rout[lr];
-          const int64_t ix = rd.off + ap * rd.rs + b;
+          const int64_t ix = rd.off + (int64_t)apj[j] * rd.rs + bj[j];
           if constexpr (CPLX)
             reinterpret_cast<double2 *>(Y)[ix] = double2{acc[0][i][j][r], acc[1][i][j][r]};
           else
@@ -208,10 +188,12 @@ void launch_mpo_stage(int bm, int bn, bool cplx, const GemmTile *tiles, const Ch
     QLG_HIP(hipGetLastError());                                                                               \
     return;                                                                                                   \
   }
-  QLG_MPO(32, 64)
-  QLG_MPO(64, 64)
+  QLG_MPO(16, 128)
   QLG_MPO(32, 128)
   QLG_MPO(64, 128)
+  QLG_MPO(16, 256)
+  QLG_MPO(32, 256)
+  QLG_MPO(64, 64)
 #undef QLG_MPO
   throw Error(QLG_ERR_UNSUPPORTED, "unsupported MPO-stage tile shape");
 }

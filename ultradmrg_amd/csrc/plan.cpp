@@ -69,11 +69,8 @@ void check_pair(const Axis &a, const Axis &b, const char *what) {
 }
 
 // ---------------------------------------------------------------------------
-// Launch order of the tiles.  Workgroup b of a launch runs on XCD b % 8, and
-// each XCD has its own L2: all tiles of one task (which share A row panels
-// and B column panels) are placed on one XCD, tasks are spread over the XCDs
-// by longest-processing-time-first, and each XCD walks its tasks from the
-// most to the least expensive.  Slots past an XCD's share are empty tiles.
+// Launch order of the tiles (make_tiles_lpt): per-task tile counts and the
+// padded K of each task feed the XCD-balanced placement.
 void Stage::make_tiles(int bm_, int bn_) {
   bm = bm_;
   bn = bn_;
@@ -176,6 +173,12 @@ void validate_stage(const Stage &s, const int64_t *slot_elems) {
               std::string("task list: ") + what + " extent [" + std::to_string(first) + ", " + std::to_string(last) +
                   "] outside buffer of " + std::to_string(slot_elems[slot]) + " elements");
   };
+  // the kernels address an operand through 32-bit byte offsets from its
+  // first element (complex elements: 16 bytes)
+  auto span = [&](int64_t first, int64_t last, const char *what) {
+    if ((last - first + 1) * 16 >= (int64_t(1) << 31))
+      throw Error(QLG_ERR_UNSUPPORTED, std::string("task list: ") + what + " operand spans more than 2^31 bytes");
+  };
   for (const GemmTask &t : s.tasks) {
     QLG_CHECK(t.M > 0 && t.N > 0 && t.seg_count >= 0, "task list: empty task shape");
     if (t.c_rowtab) {
@@ -191,16 +194,20 @@ void validate_stage(const Stage &s, const int64_t *slot_elems) {
       if (g.flags & SEG_TRANS_A) {
         QLG_CHECK(g.lda >= t.M, "task list: lda < M");
         inside(g.a_slot, g.a_off, g.a_off + (int64_t)(g.K - 1) * g.lda + (t.M - 1), "A^T");
+        span(g.a_off, g.a_off + (int64_t)(g.K - 1) * g.lda + (t.M - 1), "A^T");
       } else {
         QLG_CHECK(g.lda >= g.K, "task list: lda < K");
         inside(g.a_slot, g.a_off, g.a_off + (int64_t)(t.M - 1) * g.lda + (g.K - 1), "A");
+        span(g.a_off, g.a_off + (int64_t)(t.M - 1) * g.lda + (g.K - 1), "A");
       }
       if (g.flags & SEG_TRANS_B) {
         QLG_CHECK(g.ldb >= g.K, "task list: ldb < K");
         inside(g.b_slot, g.b_off, g.b_off + (int64_t)(t.N - 1) * g.ldb + (g.K - 1), "B^T");
+        span(g.b_off, g.b_off + (int64_t)(t.N - 1) * g.ldb + (g.K - 1), "B^T");
       } else {
         QLG_CHECK(g.ldb >= t.N, "task list: ldb < N");
         inside(g.b_slot, g.b_off, g.b_off + (int64_t)(g.K - 1) * g.ldb + (t.N - 1), "B");
+        span(g.b_off, g.b_off + (int64_t)(g.K - 1) * g.ldb + (t.N - 1), "B");
       }
     }
   }
