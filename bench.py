@@ -10,9 +10,17 @@ algorithmic matvec TFLOP/s: the sum of 2*M*N*K over the sector-GEMM pairs of
 the reference's four-step schedule divided by the measured step time.
 
 Prints one JSON line (the bench contract).  ``roofline`` describes the
-dominant kernel (the first contraction stage, L . psi), timed with HIP events
-on the stream it runs on.  ``cpu_baseline`` times the oracle's numpy matvec on
-this host over a bounded sample of the same workload.
+dominant kernel (the stage with the longest average launch), timed with HIP
+events on the stream it runs on: a contraction stage against the FP64 MFMA
+peak with its share of the reference schedule's FLOPs, the MPO stage against
+HBM bandwidth with its algorithmic bytes.  ``traffic`` is the HBM bytes per
+launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters
+(scaled by a copy kernel of known size) and recorded in
+profiles/heff_traffic_hubbard_d2000.txt; it is null for other workloads.
+``cpu_baseline`` times the oracle's numpy matvec on this host over a bounded
+sample of the same workload.
+
+    python bench.py --copy-calibration   # 1 GiB device copy, for counter scaling
 """
 from __future__ import annotations
 
@@ -27,6 +35,34 @@ sys.path.insert(0, ROOT)
 
 METRIC = "eff-H matvec TFLOP/s + sweep sec at D=2000: split-BF16 vs FP64, |dE|<1e-8"
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+STAGES = ["stage1 L.psi", "stage2 MPO", "stage3 .R^T"]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_traffic_hubbard_d2000.txt")
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Plain device-to-device copy of a known size (scale for the HBM byte counters)."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "GBps": 2 * nbytes / el / 1e9}), flush=True)
+
+
+def recorded_traffic(D, precision, stage):
+    """HBM bytes per launch of `stage` from the recorded counter runs (or None)."""
+    if D != 2000 or precision != "fp64" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        rec = json.load(f)
+    v = rec.get("bytes_per_launch", {}).get(stage)
+    return float(v) if v is not None else None
 
 
 def cpu_baseline(eff, psi, budget_s=15.0):
@@ -54,11 +90,15 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--D", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the matvec is not sharded across GPUs in this build "
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
     from tests.workloads import synthetic_eff_ham
@@ -73,6 +113,7 @@ def main():
     y = H.new_vector(zero=False)
     algo, execf = H.flops()
     sflops = H.stage_flops()
+    s_algo, s_bytes = H.stage_work()
     torch.cuda.synchronize()
     for _ in range(a.warmup):
         H.matvec_into(x.data, y.data, sync=False)
@@ -94,7 +135,17 @@ def main():
     H.enable_timing(False)
     st_ms = [v / nt for v in acc]
     dom = max(range(3), key=lambda i: st_ms[i])
-    achieved = sflops[dom] / (st_ms[dom] * 1e-3) / 1e12
+    sec = st_ms[dom] * 1e-3
+    traffic = recorded_traffic(a.D, a.precision, STAGES[dom])
+    if dom == 1:   # MPO stage: HBM-bound, algorithmic bytes
+        ach = s_bytes[dom] / sec / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+    else:          # contraction stage: MFMA-bound, its share of the reference schedule's FLOPs
+        ach = s_algo[dom] / sec / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS}
+    roof.update({"traffic": traffic, "kernel": STAGES[dom], "launch_ms": st_ms[dom],
+                 "algo_gflop": s_algo[dom] / 1e9, "algo_bytes": s_bytes[dom]})
     out = {
         "metric": METRIC,
         "value": algo / (ms * 1e-3) / 1e12,
@@ -111,10 +162,9 @@ def main():
         "config": {"workload": f"two-site H_eff matvec, Hubbard 4x16 U1xU1 MPO (w=18, d=4), D={a.D}",
                    "D": a.D, "psi_elems": H.size, "algo_gflop_per_matvec": algo / 1e9,
                    "exec_gflop_per_matvec": execf / 1e9, "precision": a.precision,
-                   "stage_ms": st_ms, "stage_gflop": [f / 1e9 for f in sflops]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": ["stage1 L.psi", "stage2 MPO", "stage3 .R^T"][dom]},
+                   "stage_ms": st_ms, "stage_exec_gflop": [f / 1e9 for f in sflops],
+                   "stage_algo_gflop": [f / 1e9 for f in s_algo], "stage_algo_mb": [b / 1e6 for b in s_bytes]},
+        "roofline": roof,
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(eff, psi)

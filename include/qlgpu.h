@@ -132,6 +132,12 @@ int qlg_plan_enable_timing(qlg_plan plan, int on);
 int qlg_plan_stage_ms(qlg_plan plan, double *ms3);
 /* FLOPs each stage issues per matvec (per product term), same order. */
 int qlg_plan_stage_flops(qlg_plan plan, double *flops3);
+/* Algorithmic work of each stage, same order: its share of the reference
+ * schedule's FLOPs (the first contraction, the two MPO contractions, the
+ * last contraction of lanczos_vmps_solver_impl.h:268-272) and its HBM bytes
+ * with every operand read once and every result written once.  Either
+ * pointer may be NULL.  No counterpart in the reference (roofline input). */
+int qlg_plan_stage_work(qlg_plan plan, double *algo_flops3, double *algo_bytes3);
 
 /* out = H_eff * psi, both in the plan's vector layout (device). */
 int qlg_matvec(qlg_plan plan, const void *psi, void *out);

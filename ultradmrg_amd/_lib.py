@@ -27,7 +27,7 @@ EXPORTS = [
     "qlg_device_alloc", "qlg_device_free", "qlg_memcpy_h2d", "qlg_memcpy_d2h", "qlg_memcpy_d2d",
     "qlg_plan_two_site", "qlg_plan_one_site", "qlg_plan_destroy", "qlg_plan_set_precision",
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
-    "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
+    "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
     "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
     "qlg_linear_combine", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_tridiag_gs", "qlg_tridiag_expme1",
 ]
@@ -89,6 +89,7 @@ def lib():
         "qlg_plan_enable_timing": (C.c_int, [vp, C.c_int]),
         "qlg_plan_stage_ms": (C.c_int, [vp, dp]),
         "qlg_plan_stage_flops": (C.c_int, [vp, dp]),
+        "qlg_plan_stage_work": (C.c_int, [vp, dp, dp]),
         "qlg_matvec": (C.c_int, [vp, vp, vp]),
         "qlg_plan_contract": (C.c_int, [vp, C.POINTER(Bsten), C.POINTER(Bsten), C.c_int, C.c_int, C.c_int,
                                         C.POINTER(vp)]),

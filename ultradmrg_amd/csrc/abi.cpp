@@ -222,6 +222,15 @@ int qlg_plan_stage_flops(qlg_plan plan, double *flops3) {
   return QLG_OK;
 }
 
+int qlg_plan_stage_work(qlg_plan plan, double *algo_flops3, double *algo_bytes3) {
+  if (!plan) return QLG_ERR_INVALID;
+  for (int i = 0; i < 3; ++i) {
+    if (algo_flops3) algo_flops3[i] = plan->p->st[i].algo_flops;
+    if (algo_bytes3) algo_bytes3[i] = plan->p->st[i].algo_bytes;
+  }
+  return QLG_OK;
+}
+
 int qlg_matvec(qlg_plan plan, const void *psi, void *out) {
   if (!plan || !psi || !out) return QLG_ERR_INVALID;
   Plan &p = *plan->p;

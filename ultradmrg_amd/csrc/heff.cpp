@@ -82,7 +82,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   // ---- symbolic reference schedule: closure of the block set + FLOPs --------
   auto symbolic = [&](const std::set<Coords> &in, double *flops) {
     std::set<Coords> out;
-    double f = 0;
+    double f = 0, f1 = 0, f4 = 0;
     // stage T1: (a', w, s..., b)
     std::set<Coords> t1;
     for (const Coords &c : in) {
@@ -97,7 +97,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         t[1] = l[1];
         for (int i = 1; i < rk; ++i) t[1 + i] = c[i];
         t1.insert(t);
-        f += fl * double(L.ax[0].deg[l[0]] * L.ax[1].deg[l[1]]) * double(L.ax[2].deg[l[2]]) * double(nfree);
+        f1 += fl * double(L.ax[0].deg[l[0]] * L.ax[1].deg[l[1]]) * double(L.ax[2].deg[l[2]]) * double(nfree);
       }
     }
     // MPO stages: carry (a', [s' done...], w_cur, [s remaining...], b)
@@ -139,10 +139,14 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         for (int j = 0; j < nphys; ++j) o[1 + j] = c[1 + j];
         o[bax] = R.blk[rb][0];
         out.insert(o);
-        f += fl * double(m) * double(kd) * double(R.ax[0].deg[R.blk[rb][0]]);
+        f4 += fl * double(m) * double(kd) * double(R.ax[0].deg[R.blk[rb][0]]);
       }
     }
-    if (flops) *flops = f;
+    if (flops) {  // [S1, S2 (+S3), S4] of the reference schedule
+      flops[0] = f1;
+      flops[1] = f;
+      flops[2] = f4;
+    }
     return out;
   };
 
@@ -153,7 +157,12 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     vset.insert(out.begin(), out.end());
     if (vset.size() == before) break;
   }
-  symbolic(vset, &p.algo_flops);
+  {
+    double f3[3];
+    symbolic(vset, f3);
+    p.algo_flops = f3[0] + f3[1] + f3[2];
+    for (int i = 0; i < 3; ++i) p.st[i].algo_flops = f3[i];
+  }
 
   // ---- vector layout ----------------------------------------------------------
   p.vec.rank = rk;
@@ -616,6 +625,13 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     launch_block_copy(ed == 2, dr.as<CopyDesc>(), (int)rcopy.size(), static_cast<const double *>(R.data),
                       p.Rint.as<double>(), ctx->stream);
     QLG_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  {
+    // algorithmic HBM bytes per launch: operands read once, results written once
+    const double e8 = 8.0 * ed;
+    p.st[0].algo_bytes = e8 * double(lint_elems + p.vec.size + p.x_elems);
+    p.st[1].algo_bytes = e8 * double(p.x_elems + p.c_elems + p.y_elems);
+    p.st[2].algo_bytes = e8 * double(p.y_elems + rint_elems + p.vec.size);
   }
   if (const char *v = std::getenv("QLG_VERBOSE"); v && v[0] == '1') {
     const char *names[3] = {"L.psi", "MPO", ".R^T"};

@@ -155,21 +155,27 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   }
 }
 
-// 2-D block copies: dst[r * dld + c] = src[r * sld + c] for each descriptor
+// 2-D block copies: dst[r * dld + c] = src[r * sld + c] for each descriptor.
+// Workgroup (x, y) copies rows y, y + kCopySplit, ... of descriptor x; each
+// wave takes one row at a time and its lanes walk the columns, so loads and
+// stores are contiguous runs and no per-element division is needed.
+constexpr int kCopySplit = 8;
+
 template <bool CPLX>
 __global__ __launch_bounds__(256) void block_copy_kernel(const CopyDesc *__restrict__ d, const double *__restrict__ src,
                                                          double *__restrict__ dst, int n) {
   const int i = blockIdx.x;
   if (i >= n) return;
   const CopyDesc c = d[i];
-  const int64_t tot = (int64_t)c.rows * c.cols;
-  for (int64_t e = threadIdx.x; e < tot; e += 256) {
-    const int64_t r = e / c.cols, col = e % c.cols;
-    if constexpr (CPLX)
-      reinterpret_cast<double2 *>(dst)[c.dst + r * c.dld + col] =
-          reinterpret_cast<const double2 *>(src)[c.src + r * c.sld + col];
-    else
-      dst[c.dst + r * c.dld + col] = src[c.src + r * c.sld + col];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = blockIdx.y + kCopySplit * wave; r < c.rows; r += kCopySplit * 4) {
+    const int64_t so = c.src + (int64_t)r * c.sld, dof = c.dst + (int64_t)r * c.dld;
+    for (int col = lane; col < c.cols; col += 64) {
+      if constexpr (CPLX)
+        reinterpret_cast<double2 *>(dst)[dof + col] = reinterpret_cast<const double2 *>(src)[so + col];
+      else
+        dst[dof + col] = src[so + col];
+    }
   }
 }
 
@@ -201,9 +207,9 @@ void launch_mpo_stage(int bm, int bn, bool cplx, const GemmTile *tiles, const Ch
 void launch_block_copy(bool cplx, const CopyDesc *d, int n, const double *src, double *dst, hipStream_t st) {
   if (n <= 0) return;
   if (cplx)
-    hipLaunchKernelGGL(block_copy_kernel<true>, dim3(n), dim3(256), 0, st, d, src, dst, n);
+    hipLaunchKernelGGL(block_copy_kernel<true>, dim3(n, kCopySplit), dim3(256), 0, st, d, src, dst, n);
   else
-    hipLaunchKernelGGL(block_copy_kernel<false>, dim3(n), dim3(256), 0, st, d, src, dst, n);
+    hipLaunchKernelGGL(block_copy_kernel<false>, dim3(n, kCopySplit), dim3(256), 0, st, d, src, dst, n);
   QLG_HIP(hipGetLastError());
 }
 

@@ -88,34 +88,63 @@ void Stage::make_tiles(int bm_, int bn_) {
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin in blockIdx order, and
-// each XCD has its own L2.  Tiles are grouped into chunks of up to kChunk
-// consecutive n-tiles of one task row (they share that row's A panel, so the
-// chunk reuses it from one L2); the chunks are placed longest-first on the
-// least-loaded XCD, and XCD x's j-th tile is launched as block j * 8 + x.
+// each XCD has its own L2.  A task's tiles share its A row panels and B
+// column panels, so a task stays on one XCD unless it is large: a task whose
+// cost exceeds 1/32 of the stage is cut into bands of whole tile rows (or
+// whole tile columns, if it has more column than row tiles), so each band
+// fetches its own A rows (B columns) once and the other operand at most once
+// per band.  Units are placed longest-first on the least-loaded XCD, tiles
+// of a unit in row-major order, and XCD x's j-th tile is launched as block
+// j * 8 + x.
 void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn) {
-  constexpr int kXcd = 8, kChunk = 8;
+  constexpr int kXcd = 8, kUnitsPerXcd = 4;
   tiles.clear();
-  struct Chunk {
-    int32_t task, m0, n0, count;
+  struct Unit {
+    int32_t task;
+    int64_t r0, r1, c0, c1;   // tile-row / tile-column ranges
     double cost;
   };
-  std::vector<Chunk> chunks;
+  std::vector<double> tile_cost(mnk.size());
+  std::vector<int64_t> ntm(mnk.size()), ntn(mnk.size());
+  double total = 0;
   for (size_t t = 0; t < mnk.size(); ++t) {
-    const int64_t ntn = (mnk[t][1] + bn - 1) / bn;
-    const double tile_cost = double(bm) * bn * std::max<int64_t>(mnk[t][2], 16) + 4096.0;
-    for (int64_t m0 = 0; m0 < mnk[t][0]; m0 += bm)
-      for (int64_t j = 0; j < ntn; j += kChunk) {
-        const int32_t c = (int32_t)std::min<int64_t>(kChunk, ntn - j);
-        chunks.push_back(Chunk{(int32_t)t, (int32_t)m0, (int32_t)(j * bn), c, c * tile_cost});
-      }
+    ntm[t] = (mnk[t][0] + bm - 1) / bm;
+    ntn[t] = (mnk[t][1] + bn - 1) / bn;
+    tile_cost[t] = double(bm) * bn * std::max<int64_t>(mnk[t][2], 16) + 4096.0;
+    total += tile_cost[t] * double(ntm[t]) * double(ntn[t]);
   }
-  std::stable_sort(chunks.begin(), chunks.end(), [](const Chunk &a, const Chunk &b) { return a.cost > b.cost; });
+  const double target = total / (kXcd * kUnitsPerXcd);
+  std::vector<Unit> units;
+  for (size_t t = 0; t < mnk.size(); ++t) {
+    const double cost = tile_cost[t] * double(ntm[t]) * double(ntn[t]);
+    if (ntm[t] * ntn[t] == 0) continue;
+    const int64_t pieces = std::max<int64_t>(1, (int64_t)std::ceil(cost / target));
+    const bool by_rows = ntm[t] >= ntn[t];
+    const int64_t len = by_rows ? ntm[t] : ntn[t];
+    const int64_t np = std::min(pieces, len);
+    for (int64_t q = 0; q < np; ++q) {
+      const int64_t lo = len * q / np, hi = len * (q + 1) / np;
+      Unit u{(int32_t)t, 0, ntm[t], 0, ntn[t], 0.0};
+      if (by_rows) {
+        u.r0 = lo;
+        u.r1 = hi;
+      } else {
+        u.c0 = lo;
+        u.c1 = hi;
+      }
+      u.cost = tile_cost[t] * double(u.r1 - u.r0) * double(u.c1 - u.c0);
+      units.push_back(u);
+    }
+  }
+  std::stable_sort(units.begin(), units.end(), [](const Unit &a, const Unit &b) { return a.cost > b.cost; });
   std::vector<std::vector<GemmTile>> per(kXcd);
   std::vector<double> load(kXcd, 0.0);
-  for (const Chunk &c : chunks) {
+  for (const Unit &u : units) {
     const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
-    load[x] += c.cost;
-    for (int32_t i = 0; i < c.count; ++i) per[x].push_back(GemmTile{c.task, c.m0, c.n0 + i * bn, 0});
+    load[x] += u.cost;
+    for (int64_t r = u.r0; r < u.r1; ++r)
+      for (int64_t c = u.c0; c < u.c1; ++c)
+        per[x].push_back(GemmTile{u.task, (int32_t)(r * bm), (int32_t)(c * bn), 0});
   }
   size_t slots = 0;
   for (auto &v : per) slots = std::max(slots, v.size());

@@ -29,6 +29,8 @@ struct Stage {
   DevBuf d_tiles, d_tasks, d_segs, d_crows, d_chan, d_rows;
   int bm = 64, bn = 64;
   double flops = 0;      // useful 2*M*N*K issued by this stage
+  double algo_flops = 0; // this stage's share of the reference schedule's FLOPs
+  double algo_bytes = 0; // operands read once + results written once (bytes)
   void upload(hipStream_t st);
   void make_tiles(int bm_, int bn_);
 };

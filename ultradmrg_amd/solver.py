@@ -126,6 +126,12 @@ class EffectiveHamiltonian:
         self.ctx.check(self.lib.qlg_plan_stage_flops(self.h, out))
         return list(out)
 
+    def stage_work(self):
+        """(algorithmic FLOPs, algorithmic HBM bytes) per stage of one matvec."""
+        f, b = (C.c_double * 3)(), (C.c_double * 3)()
+        self.ctx.check(self.lib.qlg_plan_stage_work(self.h, f, b))
+        return list(f), list(b)
+
     def stage_ms(self):
         out = (C.c_double * 3)()
         self.ctx.check(self.lib.qlg_plan_stage_ms(self.h, out))
