@@ -12,9 +12,10 @@ Reference behaviour followed:
 
 Everything stays on the GPU: contractions and environment updates go through
 the library's planners, the Krylov solve through ``qlg_lanczos_gs``, and the
-per-sector SVDs through torch (rocSOLVER).  The truncation rule is this
-build's: pooled singular values, smallest D whose discarded weight
-sum(s^2)/total <= trunc_err, clamped to [Dmin, Dmax].
+per-sector SVDs through batched FP64 eigen-decompositions of the Gram matrix
+on each sector matrix's smaller side (torch / rocSOLVER).  The truncation
+rule is this build's: pooled singular values, smallest D whose discarded
+weight sum(s^2)/total <= trunc_err, clamped to [Dmin, Dmax].
 """
 from __future__ import annotations
 
