@@ -42,11 +42,19 @@ extern "C" {
 #define QLG_F64 0   /* double                                            */
 #define QLG_C128 1  /* complex double, interleaved (re, im)              */
 
-/* Arithmetic of the sector GEMMs.  Results, the Krylov recurrence, the
- * tridiagonal solve and all reductions stay FP64 in every mode. */
-#define QLG_PREC_F64 0     /* FP64 MFMA                                    */
-#define QLG_PREC_BF16X3 2  /* operands split in 3 bf16 terms, 3 products  */
-#define QLG_PREC_BF16X6 3  /* operands split in 3 bf16 terms, 6 products  */
+/* Arithmetic of the sector GEMMs (real data).  The Krylov vectors, the
+ * recurrence, the tridiagonal solve and all reductions stay FP64 in every
+ * mode; the reduced modes keep the H_eff intermediates in FP32 and
+ * accumulate in FP32.  qlg_lanczos_gs in a reduced mode reports the FP64
+ * Rayleigh quotient of its ground state (one FP64 matvec), so the energy
+ * error is second order in the eigenvector error.  Complex plans support
+ * QLG_PREC_F64 only. */
+#define QLG_PREC_F64 0     /* FP64 MFMA                                          */
+#define QLG_PREC_F32 1     /* FP32 operands, FP32 MFMA                           */
+#define QLG_PREC_BF16X3 2  /* operands split in 2 bf16 terms (hi + lo), products
+                              hi.hi + hi.lo + lo.hi on bf16 MFMA (~16-bit)     */
+#define QLG_PREC_BF16X6 3  /* operands split in 3 bf16 terms, the 6 products of
+                              index sum <= 2 on bf16 MFMA (~24-bit, FP32-like) */
 
 #define QLG_MAX_RANK 8
 #define QLG_MAX_QN_LEN 2

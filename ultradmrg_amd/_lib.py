@@ -15,6 +15,7 @@ QLG_OK = 0
 QLG_F64 = 0
 QLG_C128 = 1
 QLG_PREC_F64 = 0
+QLG_PREC_F32 = 1
 QLG_PREC_BF16X3 = 2
 QLG_PREC_BF16X6 = 3
 

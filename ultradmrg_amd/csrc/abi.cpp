@@ -124,13 +124,21 @@ int qlg_memcpy_d2d(qlg_ctx ctx, void *dst, const void *src, int64_t bytes) {
   return copy_sync(ctx, dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
+// QLG_PREC_* values; the reduced modes exist for real data only
+static void check_precision(int precision, int dtype) {
+  QLG_CHECK(precision == QLG_PREC_F64 || precision == QLG_PREC_F32 || precision == QLG_PREC_BF16X3 ||
+                precision == QLG_PREC_BF16X6,
+            "unknown precision");
+  if (precision != QLG_PREC_F64 && dtype != QLG_F64)
+    throw Error(QLG_ERR_UNSUPPORTED, "reduced-precision sector GEMMs are implemented for real data only");
+}
+
 static int make_plan(qlg_ctx ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
                      const qlg_bsten *psi, int precision, qlg_plan *out) {
   if (!ctx || !out) return QLG_ERR_INVALID;
   *out = nullptr;
   return guarded(&ctx->c, [&] {
-    QLG_CHECK(precision == QLG_PREC_F64 || precision == QLG_PREC_BF16X3 || precision == QLG_PREC_BF16X6,
-              "unknown precision");
+    check_precision(precision, psi ? psi->dtype : QLG_F64);
     auto h = std::make_unique<qlg_plan_s>();
     h->p.reset(build_plan(&ctx->c, nphys, L, W, R, psi, precision));
     *out = h.release();
@@ -160,8 +168,8 @@ int qlg_plan_destroy(qlg_plan plan) {
 int qlg_plan_set_precision(qlg_plan plan, int precision) {
   if (!plan) return QLG_ERR_INVALID;
   return guarded(plan->p->ctx, [&] {
-    QLG_CHECK(precision == QLG_PREC_F64 || precision == QLG_PREC_BF16X3 || precision == QLG_PREC_BF16X6,
-              "unknown precision");
+    QLG_CHECK(plan->p->kind == PLAN_HEFF, "precision applies to effective-Hamiltonian plans");
+    check_precision(precision, plan->p->dtype);
     plan->p->precision = precision;
   });
 }
@@ -236,7 +244,6 @@ int qlg_matvec(qlg_plan plan, const void *psi, void *out) {
   Plan &p = *plan->p;
   return guarded(p.ctx, [&] {
     QLG_CHECK(p.kind == PLAN_HEFF, "qlg_matvec needs an effective-Hamiltonian plan");
-    QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
     p.matvec(static_cast<const double *>(psi), static_cast<double *>(out));
   });
 }
@@ -334,7 +341,7 @@ int qlg_lanczos_gs(qlg_plan plan, void *psi_inout, const qlg_lanczos_params *par
   if (!plan || !psi_inout || !params || !res) return QLG_ERR_INVALID;
   Plan &p = *plan->p;
   return guarded(p.ctx, [&] {
-    QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
+    QLG_CHECK(p.kind == PLAN_HEFF, "qlg_lanczos_gs needs an effective-Hamiltonian plan");
     lanczos_gs(p, static_cast<double *>(psi_inout), *params, *res);
   });
 }
@@ -343,7 +350,9 @@ int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau, const qlg_lanc
   if (!plan || !psi_inout || !params || !iters) return QLG_ERR_INVALID;
   Plan &p = *plan->p;
   return guarded(p.ctx, [&] {
-    QLG_CHECK(p.precision == QLG_PREC_F64, "split-bf16 sector GEMMs are not available in this build");
+    QLG_CHECK(p.kind == PLAN_HEFF, "qlg_lanczos_expmv needs an effective-Hamiltonian plan");
+    if (p.precision != QLG_PREC_F64)
+      throw Error(QLG_ERR_UNSUPPORTED, "qlg_lanczos_expmv runs in FP64 (complex data)");
     lanczos_expmv(p, static_cast<double *>(psi_inout), tau, *params, *iters);
   });
 }

@@ -1,0 +1,415 @@
+// Reduced-precision grouped GEMMs for the real effective-Hamiltonian matvec.
+//
+// Same task / segment / tile lists as gemm_f64.hip.  A operands are FP32 in
+// memory (the plan keeps FP32 copies of the environment stacks, and the
+// intermediates are FP32); B operands are FP64 (the Krylov vector) or FP32;
+// the output is FP32 (intermediate) or FP64 (the Krylov output, written
+// through a row-address table).  Accumulation is FP32 in the MFMA units.
+//
+//  * gemm_f32_kernel: v_mfma_f32_16x16x4_f32 on FP32 operands.
+//  * gemm_bf16_kernel: split-BF16.  Each FP32 operand value x is split into
+//    bf16 pieces h = bf16(x), m = bf16(x - h), l = bf16(x - h - m) (each
+//    subtraction exact in FP32) and the product is formed from the piece
+//    products on v_mfma_f32_16x16x32_bf16:
+//      NPC = 3:  h.h + h.l' + l'.h  with two pieces (l' = bf16(x - h)),
+//                ~16 significant bits per operand;
+//      NPC = 6:  h.h + h.m + m.h + h.l + l.h + m.m  with three pieces,
+//                ~24 bits (FP32-like).
+//    Operands are staged in LDS as FP32 and split when the MFMA fragments
+//    are read, so LDS holds one copy whatever the number of pieces.
+//
+// MFMA layouts (measured on gfx950 with an in-bounds probe):
+//   f32 16x16x4:   A lane l: A[l % 16][l / 16];  B lane l: B[l / 16][l % 16]
+//   bf16 16x16x32: A lane l: A[l % 16][8 (l / 16) + j], j < 8;
+//                  B lane l: B[8 (l / 16) + j][l % 16]
+//   both:          D lane l, register r: D[4 (l / 16) + r][l % 16]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "gemm.h"
+#include "tile_io.h"
+
+namespace qlg {
+
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+__device__ inline float load_f32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+template <typename T>
+__device__ inline float load_as_f32(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
+  if constexpr (sizeof(T) == 8)
+    return (float)load_f64(r, elem == kOob ? kOob : elem * 8u);
+  else
+    return load_f32(r, elem == kOob ? kOob : elem * 4u);
+}
+
+// output row addresses (elements from the task's C base): table or stride
+template <int FM>
+__device__ inline void out_rows(const GemmTask &tk, const GemmTile &tl, const Slots &slots, int wm0, int lane,
+                                int64_t (&crow)[FM][4]) {
+  if (tk.c_rowtab) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+        crow[i][r] = slots.crows[tk.c_row + min(row, tk.M - 1)] - tk.c_off;
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) crow[i][r] = (int64_t)(tl.m0 + wm0 + i * 16 + 4 * (lane >> 4) + r) * tk.ldc;
+  }
+}
+
+template <int FM, int FN, typename TC>
+__device__ inline void store_tile(const GemmTask &tk, const GemmTile &tl, const Slots &slots, int wm0, int wn0,
+                                  int lane, const flt4 (&acc)[FM][FN]) {
+  int64_t crow[FM][4];
+  out_rows<FM>(tk, tl, slots, wm0, lane, crow);
+  TC *C = reinterpret_cast<TC *>(slots.p[tk.c_slot]) + tk.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = tl.n0 + wn0 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+        if (row < tk.M && col < tk.N) C[crow[i][r] + col] = (TC)acc[i][j][r];
+      }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA.  LDS: As[k][m], Bs[k][n] (FP32), BK-deep slices, ping-pong.
+template <int BM, int BN, int BK, typename TB, typename TC>
+__global__ __launch_bounds__(256) void gemm_f32_kernel(const GemmTile *__restrict__ tiles,
+                                                       const GemmTask *__restrict__ tasks,
+                                                       const GemmSeg *__restrict__ segs, Slots slots, int ntiles) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int A_PER = BM * BK / 256, B_PER = BK * BN / 256;
+  __shared__ float As[2][BK][BM + 1];
+  __shared__ float Bs[2][BK][BN + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  if ((int)blockIdx.x >= ntiles) return;
+  const GemmTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GemmTask tk = tasks[tl.task];
+  const int M = tk.M, N = tk.N;
+
+  flt4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+  float ra[A_PER], rb[B_PER];
+
+  auto b_kn = [&](int i, bool tb, int &k, int &n) {
+    const int e = tid + i * 256;
+    n = tb ? e / BK : e % BN;
+    k = tb ? e % BK : e / BN;
+  };
+  auto load = [&](const GemmSeg &sg, int k0) {
+    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
+    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
+    const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
+    const bool tb = sg.flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int e = tid + i * 256, m = e / BK, k = e % BK;
+      const uint32_t gm = tl.m0 + m, gk = k0 + k;
+      ra[i] = load_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      const uint32_t gn = tl.n0 + n, gk = k0 + k;
+      rb[i] = load_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+    }
+  };
+  auto store = [&](int buf, uint8_t flags) {
+    const bool tb = flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int e = tid + i * 256;
+      As[buf][e % BK][e / BK] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      Bs[buf][k][n] = rb[i];
+    }
+  };
+
+  const int s_end = tk.seg_begin + tk.seg_count;
+  int s = tk.seg_begin;
+  while (s < s_end && segs[s].K <= 0) ++s;
+  if (s < s_end) {
+    GemmSeg cur = segs[s];
+    int k0 = 0, buf = 0;
+    load(cur, 0);
+    store(0, cur.flags);
+    __syncthreads();
+    while (true) {
+      int ns = s, nk = k0 + BK;
+      GemmSeg nxt = cur;
+      if (nk >= cur.K) {
+        ++ns;
+        nk = 0;
+        while (ns < s_end && segs[ns].K <= 0) ++ns;
+        if (ns < s_end) nxt = segs[ns];
+      }
+      const bool more = ns < s_end;
+      if (more) load(nxt, nk);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        float af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = As[buf][kk + (lane >> 4)][wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+      }
+      if (!more) break;
+      store(buf ^ 1, nxt.flags);
+      __syncthreads();
+      buf ^= 1;
+      s = ns;
+      k0 = nk;
+      cur = nxt;
+    }
+  }
+  store_tile<FM, FN, TC>(tk, tl, slots, wm0, wn0, lane, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16.  LDS: As[m][k], Bs[n][k] (FP32, k contiguous, 32-deep slices),
+// ping-pong; fragments are split into bf16 pieces in registers.
+namespace {
+constexpr int BKB = 32;
+constexpr int KP = BKB + 4;   // row pitch (floats)
+
+template <int NPIECE>
+__device__ inline void split8(const float (&x)[8], bf16x8 (&p)[NPIECE]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float r = x[j];
+#pragma unroll
+    for (int q = 0; q < NPIECE; ++q) {
+      const __bf16 h = (__bf16)r;
+      p[q][j] = h;
+      r -= (float)h;
+    }
+  }
+}
+}  // namespace
+
+template <int BM, int BN, int NPC, typename TB, typename TC>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmTile *__restrict__ tiles,
+                                                        const GemmTask *__restrict__ tasks,
+                                                        const GemmSeg *__restrict__ segs, Slots slots, int ntiles) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int A_PER = BM * BKB / 256, B_PER = BKB * BN / 256;
+  constexpr int NPIECE = NPC == 3 ? 2 : 3;
+  __shared__ float As[2][BM][KP];
+  __shared__ float Bs[2][BN][KP];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  if ((int)blockIdx.x >= ntiles) return;
+  const GemmTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GemmTask tk = tasks[tl.task];
+  const int M = tk.M, N = tk.N;
+
+  flt4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+  float ra[A_PER], rb[B_PER];
+
+  // B element e: consecutive threads read consecutive addresses in either storage order
+  auto b_kn = [&](int i, bool tb, int &k, int &n) {
+    const int e = tid + i * 256;
+    n = tb ? e / BKB : e % BN;
+    k = tb ? e % BKB : e / BN;
+  };
+  auto load = [&](const GemmSeg &sg, int k0) {
+    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
+    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
+    const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
+    const bool tb = sg.flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int e = tid + i * 256, m = e / BKB, k = e % BKB;
+      const uint32_t gm = tl.m0 + m, gk = k0 + k;
+      ra[i] = load_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      const uint32_t gn = tl.n0 + n, gk = k0 + k;
+      rb[i] = load_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+    }
+  };
+  auto store = [&](int buf, uint8_t flags) {
+    const bool tb = flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int e = tid + i * 256;
+      As[buf][e / BKB][e % BKB] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      Bs[buf][n][k] = rb[i];
+    }
+  };
+  auto step = [&](int buf) {
+    bf16x8 ap[FM][NPIECE], bp[FN][NPIECE];
+    const int k8 = 8 * (lane >> 4);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      float x[8];
+      const float *src = &As[buf][wm0 + i * 16 + (lane & 15)][k8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = src[j];
+      split8<NPIECE>(x, ap[i]);
+    }
+#pragma unroll
+    for (int jn = 0; jn < FN; ++jn) {
+      float x[8];
+      const float *src = &Bs[buf][wn0 + jn * 16 + (lane & 15)][k8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = src[j];
+      split8<NPIECE>(x, bp[jn]);
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        flt4 c = acc[i][j];
+        // small products first
+        if constexpr (NPC == 6) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][1], bp[j][1], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][2], bp[j][0], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][2], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][1], bp[j][0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][0], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+  };
+
+  const int s_end = tk.seg_begin + tk.seg_count;
+  int s = tk.seg_begin;
+  while (s < s_end && segs[s].K <= 0) ++s;
+  if (s < s_end) {
+    GemmSeg cur = segs[s];
+    int k0 = 0, buf = 0;
+    load(cur, 0);
+    store(0, cur.flags);
+    __syncthreads();
+    while (true) {
+      int ns = s, nk = k0 + BKB;
+      GemmSeg nxt = cur;
+      if (nk >= cur.K) {
+        ++ns;
+        nk = 0;
+        while (ns < s_end && segs[ns].K <= 0) ++ns;
+        if (ns < s_end) nxt = segs[ns];
+      }
+      const bool more = ns < s_end;
+      if (more) load(nxt, nk);
+      step(buf);
+      if (!more) break;
+      store(buf ^ 1, nxt.flags);
+      __syncthreads();
+      buf ^= 1;
+      s = ns;
+      k0 = nk;
+      cur = nxt;
+    }
+  }
+  store_tile<FM, FN, TC>(tk, tl, slots, wm0, wn0, lane, acc);
+}
+
+// ---------------------------------------------------------------------------
+// launchers.  b_f64: B operand FP64 (else FP32); c_f64: output FP64 (else FP32).
+template <typename TB, typename TC>
+static void launch_mixed_typed(int mode, int bm, int bn, const GemmTile *tiles, const GemmTask *tasks,
+                               const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st) {
+  dim3 grid(ntiles), block(256);
+#define QLG_F32(M_, N_)                                                                                       \
+  if (bm == M_ && bn == N_) {                                                                                 \
+    hipLaunchKernelGGL((gemm_f32_kernel<M_, N_, 16, TB, TC>), grid, block, 0, st, tiles, tasks, segs, slots,  \
+                       ntiles);                                                                               \
+    return;                                                                                                   \
+  }
+#define QLG_BF(M_, N_, P_)                                                                                    \
+  if (bm == M_ && bn == N_) {                                                                                 \
+    hipLaunchKernelGGL((gemm_bf16_kernel<M_, N_, P_, TB, TC>), grid, block, 0, st, tiles, tasks, segs, slots, \
+                       ntiles);                                                                               \
+    return;                                                                                                   \
+  }
+  if (mode == QLG_PREC_F32) {
+    QLG_F32(32, 32) QLG_F32(64, 32) QLG_F32(32, 64) QLG_F32(64, 64) QLG_F32(128, 64) QLG_F32(64, 128)
+    QLG_F32(128, 128)
+  } else if (mode == QLG_PREC_BF16X3) {
+    QLG_BF(32, 32, 3) QLG_BF(64, 32, 3) QLG_BF(32, 64, 3) QLG_BF(64, 64, 3) QLG_BF(128, 64, 3)
+    QLG_BF(64, 128, 3) QLG_BF(128, 128, 3)
+  } else if (mode == QLG_PREC_BF16X6) {
+    QLG_BF(32, 32, 6) QLG_BF(64, 32, 6) QLG_BF(32, 64, 6) QLG_BF(64, 64, 6) QLG_BF(128, 64, 6)
+    QLG_BF(64, 128, 6) QLG_BF(128, 128, 6)
+  }
+#undef QLG_F32
+#undef QLG_BF
+  throw Error(QLG_ERR_UNSUPPORTED, "unsupported reduced-precision tile shape or mode");
+}
+
+__global__ __launch_bounds__(256) void f64_to_f32_kernel(const double *__restrict__ src, float *__restrict__ dst,
+                                                         int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    dst[i] = (float)src[i];
+}
+
+void launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, n);
+  QLG_HIP(hipGetLastError());
+}
+
+void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, const GemmTile *tiles,
+                               const GemmTask *tasks, const GemmSeg *segs, const Slots &slots, int ntiles,
+                               hipStream_t st) {
+  if (ntiles <= 0) return;
+  if (b_f64 && !c_f64)
+    launch_mixed_typed<double, float>(mode, bm, bn, tiles, tasks, segs, slots, ntiles, st);
+  else if (!b_f64 && c_f64)
+    launch_mixed_typed<float, double>(mode, bm, bn, tiles, tasks, segs, slots, ntiles, st);
+  else
+    throw Error(QLG_ERR_UNSUPPORTED, "reduced-precision GEMM operand combination");
+  QLG_HIP(hipGetLastError());
+}
+
+}  // namespace qlg

@@ -608,6 +608,8 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   p.C.ensure(std::max<int64_t>(p.c_elems, 1) * eb);
   p.Lint.ensure(std::max<int64_t>(lint_elems, 1) * eb);
   p.Rint.ensure(std::max<int64_t>(rint_elems, 1) * eb);
+  p.lint_elems = lint_elems;
+  p.rint_elems = rint_elems;
   // Y entries no channel writes stay zero for the lifetime of the plan
   QLG_HIP(hipMemsetAsync(p.Y.p, 0, std::max<int64_t>(p.y_elems, 1) * eb, ctx->stream));
   if (p.c_elems > 0)
@@ -678,15 +680,31 @@ void Plan::to_canonical(const double *fused, double *canon) {
   launch_block_copy(dtype == QLG_C128, d_pack_out.as<CopyDesc>(), (int)pack_out.size(), fused, canon, ctx->stream);
 }
 
+void Plan::prepare_reduced() {
+  if (reduced_ready) return;
+  QLG_CHECK(dtype == QLG_F64, "reduced-precision modes need real data");
+  Lint32.ensure(std::max<int64_t>(lint_elems, 1) * sizeof(float));
+  Rint32.ensure(std::max<int64_t>(rint_elems, 1) * sizeof(float));
+  C32.ensure(std::max<int64_t>(c_elems, 1) * sizeof(float));
+  Y32.ensure(std::max<int64_t>(y_elems, 1) * sizeof(float));
+  QLG_HIP(hipMemsetAsync(Y32.p, 0, std::max<int64_t>(y_elems, 1) * sizeof(float), ctx->stream));
+  launch_f64_to_f32(Lint.as<double>(), Lint32.as<float>(), lint_elems, ctx->stream);
+  launch_f64_to_f32(Rint.as<double>(), Rint32.as<float>(), rint_elems, ctx->stream);
+  launch_f64_to_f32(C.as<double>(), C32.as<float>(), c_elems, ctx->stream);
+  reduced_ready = true;
+}
+
 void Plan::matvec_fused(const double *in, double *out) {
+  const bool reduced = precision != QLG_PREC_F64;
+  if (reduced) prepare_reduced();
   Slots s{};
   s.p[SLOT_IN] = const_cast<double *>(in);
   s.p[SLOT_OUT] = out;
-  s.p[SLOT_L] = Lint.as<double>();
-  s.p[SLOT_R] = Rint.as<double>();
+  s.p[SLOT_L] = reduced ? reinterpret_cast<double *>(Lint32.p) : Lint.as<double>();
+  s.p[SLOT_R] = reduced ? reinterpret_cast<double *>(Rint32.p) : Rint.as<double>();
   s.p[SLOT_X] = X.as<double>();
-  s.p[SLOT_Y] = Y.as<double>();
-  s.p[SLOT_C] = C.as<double>();
+  s.p[SLOT_Y] = reduced ? reinterpret_cast<double *>(Y32.p) : Y.as<double>();
+  s.p[SLOT_C] = reduced ? reinterpret_cast<double *>(C32.p) : C.as<double>();
   s.crows = st[2].d_crows.as<int64_t>();
   hipStream_t stream = ctx->stream;
   const bool cplx = dtype == QLG_C128;
@@ -695,17 +713,27 @@ void Plan::matvec_fused(const double *in, double *out) {
       if (!e) QLG_HIP(hipEventCreate(&e));
     QLG_HIP(hipEventRecord(ev[0], stream));
   }
+  // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1
   Stage &a = st[0];
-  launch_grouped_gemm_f64(a.bm, a.bn, cplx, a.d_tiles.as<GemmTile>(), a.d_tasks.as<GemmTask>(),
-                          a.d_segs.as<GemmSeg>(), s, (int)a.tiles.size(), stream);
+  if (reduced)
+    launch_grouped_gemm_mixed(precision, a.bm, a.bn, true, false, a.d_tiles.as<GemmTile>(), a.d_tasks.as<GemmTask>(),
+                              a.d_segs.as<GemmSeg>(), s, (int)a.tiles.size(), stream);
+  else
+    launch_grouped_gemm_f64(a.bm, a.bn, cplx, a.d_tiles.as<GemmTile>(), a.d_tasks.as<GemmTask>(),
+                            a.d_segs.as<GemmSeg>(), s, (int)a.tiles.size(), stream);
   if (timing) QLG_HIP(hipEventRecord(ev[1], stream));
   Stage &b = st[1];
-  launch_mpo_stage(b.bm, b.bn, cplx, b.d_tiles.as<GemmTile>(), b.d_chan.as<ChanTask>(), b.d_rows.as<RowDesc>(),
-                   C.as<double>(), X.as<double>(), Y.as<double>(), (int)b.tiles.size(), stream);
+  launch_mpo_stage(b.bm, b.bn, cplx, reduced, b.d_tiles.as<GemmTile>(), b.d_chan.as<ChanTask>(),
+                   b.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], (int)b.tiles.size(), stream);
   if (timing) QLG_HIP(hipEventRecord(ev[2], stream));
+  // stage 3: A = Y, B = R panels (transposed), C = the Krylov output (FP64, row table)
   Stage &c = st[2];
-  launch_grouped_gemm_f64(c.bm, c.bn, cplx, c.d_tiles.as<GemmTile>(), c.d_tasks.as<GemmTask>(),
-                          c.d_segs.as<GemmSeg>(), s, (int)c.tiles.size(), stream);
+  if (reduced)
+    launch_grouped_gemm_mixed(precision, c.bm, c.bn, false, true, c.d_tiles.as<GemmTile>(), c.d_tasks.as<GemmTask>(),
+                              c.d_segs.as<GemmSeg>(), s, (int)c.tiles.size(), stream);
+  else
+    launch_grouped_gemm_f64(c.bm, c.bn, cplx, c.d_tiles.as<GemmTile>(), c.d_tasks.as<GemmTask>(),
+                            c.d_segs.as<GemmSeg>(), s, (int)c.tiles.size(), stream);
   if (timing) QLG_HIP(hipEventRecord(ev[3], stream));
 }
 

@@ -70,6 +70,24 @@ struct Workspace {
     // the host arrays above must stay alive until the copies are done
     QLG_HIP(hipStreamSynchronize(c.stream));
   }
+  // FP64 Rayleigh quotient <g|H|g> / <g|g> of the fused vector the last
+  // combine() formed (the basis slots are free by then).  Used by the
+  // reduced-precision modes: the energy error becomes second order in the
+  // error of the ground-state vector.
+  double rayleigh_f64() {
+    Ctx &c = *p.ctx;
+    const double *g = p.vtmp_out.as<double>();
+    const int prec = p.precision;
+    p.precision = QLG_PREC_F64;
+    p.matvec_fused(g, slot(0));
+    p.precision = prec;
+    vec_dot(c, p.dtype, n, slot(0), g, dev_tmp);
+    vec_nrm2(c, p.dtype, n, g, dev_tmp + 1);
+    double h[2];
+    fetch(dev_tmp, h, 2);
+    QLG_CHECK(h[1] > 0.0, "ground state has zero norm");
+    return h[0] / (h[1] * h[1]);
+  }
 };
 
 }  // namespace
@@ -106,6 +124,12 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
     if (norm_gamma == 0.0) {
       if (m == 1) {
         p.to_canonical(w.slot(0), psi);
+        if (p.precision != QLG_PREC_F64) {
+          p.vtmp_out.ensure((size_t)w.nd * sizeof(double));
+          QLG_HIP(hipMemcpyAsync(p.vtmp_out.p, w.slot(0), (size_t)w.nd * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c.stream));
+          energy0 = w.rayleigh_f64();
+        }
         QLG_HIP(hipStreamSynchronize(c.stream));
         res.iters = m;
         res.gs_eng = energy0;
@@ -117,7 +141,7 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
         throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
       w.combine(m, vec.data(), false, psi);
       res.iters = m;
-      res.gs_eng = energy0;
+      res.gs_eng = p.precision == QLG_PREC_F64 ? energy0 : w.rayleigh_f64();
       return;
     }
     N[m] = norm_gamma * norm_gamma;
@@ -142,7 +166,7 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
         throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
       w.combine(m + 1, vec.data(), false, psi);
       res.iters = m + 1;
-      res.gs_eng = e_new;
+      res.gs_eng = p.precision == QLG_PREC_F64 ? e_new : w.rayleigh_f64();
       return;
     }
     energy0 = e_new;

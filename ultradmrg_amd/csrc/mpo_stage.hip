@@ -24,25 +24,49 @@
 namespace qlg {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 namespace {
 constexpr int KC = 32;        // K chunk held in registers per round
 constexpr int kRowWin = 256;
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {   // v_mfma_f64_16x16x4: D lane l, reg r -> row l / 16 + 4 r
+  typedef dbl4 acc_t;
+  __device__ static acc_t run(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  __device__ static double load(__amdgpu_buffer_rsrc_t rs, uint32_t elem) { return load_f64(rs, elem == kOob ? kOob : elem * 8u); }
+};
+template <>
+struct Mfma<float> {    // v_mfma_f32_16x16x4: D lane l, reg r -> row 4 (l / 16) + r
+  typedef flt4 acc_t;
+  __device__ static acc_t run(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  __device__ static float load(__amdgpu_buffer_rsrc_t rs, uint32_t elem) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, elem == kOob ? kOob : elem * 4u, 0, 0));
+  }
+};
 }  // namespace
 
-template <int BM, int BN, bool CPLX>
+// T: element type of C, X and Y (double, or float for the reduced-precision
+// modes); CPLX (complex double) only with T = double.
+template <int BM, int BN, bool CPLX, typename T>
 __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restrict__ tiles,
                                                         const ChanTask *__restrict__ tasks,
-                                                        const RowDesc *__restrict__ rows, const double *__restrict__ C,
-                                                        const double *__restrict__ X, double *__restrict__ Y,
+                                                        const RowDesc *__restrict__ rows, const T *__restrict__ C,
+                                                        const T *__restrict__ X, T *__restrict__ Y,
                                                         int ntiles) {
+  static_assert(!CPLX || sizeof(T) == 8, "complex data is double");
+  using MF = Mfma<T>;
+  typedef typename MF::acc_t acc_t;
   constexpr int WN = BN / 4;                // columns per wave
   constexpr int FM = BM / 16, FN = WN / 16;
   constexpr int NP = CPLX ? 2 : 1;
-  constexpr uint32_t EB = CPLX ? 16u : 8u;
   constexpr int KS = KC / 4;                // MFMA k-steps per chunk
   static_assert(FM >= 1 && FN >= 1, "tile too small");
-  __shared__ double As[NP][KC][BM + 1];
+  __shared__ T As[NP][KC][BM + 1];
   __shared__ RowDesc rin[kRowWin];
   __shared__ RowDesc rout[BM];
 
@@ -53,7 +77,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   if (tl.task < 0) return;
   const ChanTask tk = tasks[tl.task];
   const int M = tk.kout, N = tk.P, K = tk.kin;
-  const double *Cm = C + tk.c_off * NP;
+  const T *Cm = C + tk.c_off * NP;
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(X);
   const RowDesc *gin = rows + tk.in_rows;
   for (int i = tid; i < min(K, kRowWin); i += 256) rin[i] = gin[i];
@@ -70,13 +94,13 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
     bj[j] = okj[j] ? col % tk.Db : 0;
   }
 
-  dbl4 acc[NP][FM][FN];
+  acc_t acc[NP][FM][FN];
 #pragma unroll
   for (int p = 0; p < NP; ++p)
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) acc[p][i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < FN; ++j) acc[p][i][j] = acc_t{0, 0, 0, 0};
 
   for (int kc = 0; kc < K; kc += KC) {
     __syncthreads();  // descriptors staged / previous chunk of As consumed
@@ -90,11 +114,11 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
         As[0][k][m] = ok ? v.x : 0.0;
         As[1][k][m] = ok ? v.y : 0.0;
       } else {
-        As[0][k][m] = ok ? Cm[ix] : 0.0;
+        As[0][k][m] = ok ? Cm[ix] : T(0);
       }
     }
     // B fragments of the whole chunk: lane holds X[row kc + 4 s + lane / 16][its column j]
-    double rb[NP][KS][FN];
+    T rb[NP][KS][FN];
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
       const int gk = kc + 4 * st + (lane >> 4);
@@ -102,21 +126,20 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
       const RowDesc rd = K <= kRowWin ? rin[kok ? gk : 0] : gin[kok ? gk : 0];
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        const uint32_t off =
-            (kok && okj[j]) ? (uint32_t)((int)rd.off + apj[j] * (int)rd.rs + bj[j]) * EB : kOob;
+        const uint32_t el = (kok && okj[j]) ? (uint32_t)((int)rd.off + apj[j] * (int)rd.rs + bj[j]) : kOob;
         if constexpr (CPLX) {
-          const double2 v = load_c128(xr, off);
+          const double2 v = load_c128(xr, el == kOob ? kOob : el * 16u);
           rb[0][st][j] = v.x;
           rb[1][st][j] = v.y;
         } else {
-          rb[0][st][j] = load_f64(xr, off);
+          rb[0][st][j] = MF::load(xr, el);
         }
       }
     }
     __syncthreads();  // C chunk staged
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
-      double af[NP][FM];
+      T af[NP][FM];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -125,7 +148,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], rb[0][st][j], acc[0][i][j], 0, 0, 0);
+          acc[0][i][j] = MF::run(af[0][i], rb[0][st][j], acc[0][i][j]);
           if constexpr (CPLX) {
             acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[1][i], rb[1][st][j], acc[0][i][j], 0, 0, 0);
             acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], rb[1][st][j], acc[1][i][j], 0, 0, 0);
@@ -142,7 +165,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int lr = i * 16 + (lane >> 4) + 4 * r;
+        const int lr = i * 16 + MF::row(lane, r);
         if (tl.m0 + lr < M) {
           const RowDesc rd = rout[lr];
           const int64_t ix = rd.off + (int64_t)apj[j] * rd.rs + bj[j];
@@ -179,18 +202,25 @@ __global__ __launch_bounds__(256) void block_copy_kernel(const CopyDesc *__restr
   }
 }
 
-void launch_mpo_stage(int bm, int bn, bool cplx, const GemmTile *tiles, const ChanTask *tasks, const RowDesc *rows,
-                      const double *C, const double *X, double *Y, int ntiles, hipStream_t st) {
+void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles, const ChanTask *tasks,
+                      const RowDesc *rows, const void *C, const void *X, void *Y, int ntiles, hipStream_t st) {
   if (ntiles <= 0) return;
   dim3 grid(ntiles), block(256);
+  const double *Cd = static_cast<const double *>(C), *Xd = static_cast<const double *>(X);
+  const float *Cf = static_cast<const float *>(C), *Xf = static_cast<const float *>(X);
+  double *Yd = static_cast<double *>(Y);
+  float *Yf = static_cast<float *>(Y);
 #define QLG_MPO(M_, N_)                                                                                       \
   if (bm == M_ && bn == N_) {                                                                                 \
-    if (cplx)                                                                                                 \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true>), grid, block, 0, st, tiles, tasks, rows, C, X, Y,   \
-                         ntiles);                                                                             \
+    if (f32)                                                                                                  \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float>), grid, block, 0, st, tiles, tasks, rows, Cf, \
+                         Xf, Yf, ntiles);                                                                     \
+    else if (cplx)                                                                                            \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, double>), grid, block, 0, st, tiles, tasks, rows, Cd, \
+                         Xd, Yd, ntiles);                                                                     \
     else                                                                                                      \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false>), grid, block, 0, st, tiles, tasks, rows, C, X, Y,  \
-                         ntiles);                                                                             \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, double>), grid, block, 0, st, tiles, tasks, rows,   \
+                         Cd, Xd, Yd, ntiles);                                                                 \
     QLG_HIP(hipGetLastError());                                                                               \
     return;                                                                                                   \
   }

@@ -1,16 +1,11 @@
-// Effective-Hamiltonian plan: sector pairing done once per site, three
-// grouped-GEMM stages replayed for every matvec.
+// Plans: sector pairing done once, grouped-GEMM task lists replayed.
 //
-// Stage 1  (L . psi):   for every L block (a', w, a) and state block
-//          (a, s1, s2, b) and every MPO / physical component, one GEMM
-//          X[a', b] = L[a', w, :] . psi[:, s1, s2, b]   (M=D_a', K=D_a, N=D_b),
-//          written as a contiguous chunk of the channel (a'-sector, b-sector).
-// Stage 2  (MPO):       per channel, Y = C . X with C the combined two-site MPO
-//          operator sum_y Wl[w,s1,t1,y] Wr[y,s2,t2,z] restricted to the
-//          channel's input chunks (w,s1,s2) and output rows (t1,t2,z).
-// Stage 3  (. R^T):     per output block (a', t1, t2, b'), the sum over
-//          (z, b) segments of Y[a', b] . R[b', z, b]^T.
-// The one-site form uses the same three stages with C = W.
+// The effective-Hamiltonian plan (heff.cpp) runs three launches per matvec:
+//  stage 1  per ket sector a:  T1_a = Lstack_a . Psi_a   (grouped GEMM)
+//  stage 2  per channel (a', b): Y = C . X on gathered / scattered rows of
+//           T1 and Y, C the combined two-site MPO operator  (mpo_stage.hip)
+//  stage 3  per bra sector b': out = Y_b' . R_b'^T        (grouped GEMM)
+// Tensordot and environment-update plans (ctr.cpp) reuse the grouped GEMM.
 #pragma once
 #include <vector>
 
@@ -77,6 +72,14 @@ struct Plan {
   std::vector<CopyDesc> pack_in, pack_out;
   DevBuf d_pack_in, d_pack_out;
   DevBuf Lint, Rint;            // environments repacked (ket-major L stacks, bra-major R panels)
+  int64_t lint_elems = 0, rint_elems = 0;
+  // reduced-precision modes: FP32 copies of the L stacks, R panels and
+  // channel operators (made on first use).  T1 reuses X (stage 1 rewrites
+  // all of it every matvec); Y32 is a buffer of its own because stage 3
+  // reads Y entries that no channel writes, which must stay zero.
+  DevBuf Lint32, Rint32, C32, Y32;
+  bool reduced_ready = false;
+  void prepare_reduced();
   DevBuf vtmp_in, vtmp_out;     // staging for qlg_matvec on canonical vectors
   ~Plan();
   void matvec(const double *in, double *out);        // canonical in / out
@@ -95,8 +98,12 @@ void check_pair(const Axis &a, const Axis &b, const char *what);
 std::vector<int64_t> sector_offsets(const Axis &a);
 std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed);
 void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
-void launch_mpo_stage(int bm, int bn, bool cplx, const GemmTile *tiles, const ChanTask *tasks, const RowDesc *rows,
-                      const double *C, const double *X, double *Y, int ntiles, hipStream_t st);
+void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles, const ChanTask *tasks,
+                      const RowDesc *rows, const void *C, const void *X, void *Y, int ntiles, hipStream_t st);
+void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, const GemmTile *tiles,
+                               const GemmTask *tasks, const GemmSeg *segs, const Slots &slots, int ntiles,
+                               hipStream_t st);
+void launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 void launch_block_copy(bool cplx, const CopyDesc *d, int n, const double *src, double *dst, hipStream_t st);
 void validate_stage(const Stage &s, const int64_t *slot_elems);
 void autotune_tiles(Stage &s, bool cplx);

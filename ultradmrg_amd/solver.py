@@ -31,7 +31,8 @@ import torch
 from . import _lib as L
 from .tensor import BlockTensor, Context, default_context
 
-PRECISIONS = {"fp64": L.QLG_PREC_F64, "bf16x3": L.QLG_PREC_BF16X3, "bf16x6": L.QLG_PREC_BF16X6}
+PRECISIONS = {"fp64": L.QLG_PREC_F64, "fp32": L.QLG_PREC_F32, "bf16x3": L.QLG_PREC_BF16X3,
+              "bf16x6": L.QLG_PREC_BF16X6}
 
 
 @dataclass
