@@ -21,6 +21,16 @@ profiles/heff_traffic_hubbard_d2000.txt; it is null for other workloads.
 sample of the same workload.
 
     python bench.py --copy-calibration   # 1 GiB device copy, for counter scaling
+
+Beside the FP64 matvec (``value``), the line carries the same matvec in the
+split-BF16 6-term mode (``split``) and, unless ``--no-sweep``, the sweep
+comparison of the headline metric (``sweep``): the 4x16 Hubbard cylinder
+(t=1, U=8, 56 electrons, Sz=0, the reference benchmark's initial product
+state, ``benchmark/ultradmrg_hubbard2d.cpp:130-248``) is ramped in FP64 through
+D = 10, 100, 500, 1000 (one sweep each) and three sweeps at D=2000, then two
+sweeps at D=2000 (Lanczos 1e-9 / 100, truncation 1e-8) are timed in FP64 and,
+from the same state, in the split-BF16 mode; energies are compared relative
+to FP64.
 """
 from __future__ import annotations
 
@@ -65,6 +75,68 @@ def recorded_traffic(D, precision, stage):
     return float(v) if v is not None else None
 
 
+def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 2000, 2000, 2000), D=2000,
+                             nsweeps=2):
+    """Seconds per D=2000 sweep in FP64 and in the split mode, from one ramped state."""
+    import torch
+    from ultradmrg_amd import mpo as M
+    from ultradmrg_amd.solver import LanczosParams
+    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, direct_state_mps
+
+    m = M.hubbard_cylinder(lx, 4, t=1.0, U=8.0)
+    n = m.n
+    per_hole, labs, sz = n // 8, [], 0
+    for i in range(n):   # one hole per 8 sites, alternating spins elsewhere
+        if i % per_hole == per_hole - 1:
+            labs.append(3)
+        else:
+            labs.append(sz % 2 + 1)
+            sz += 1
+    lp = LanczosParams(1e-9, 100)
+    drv = TwoSiteVMPS(direct_state_mps(m, labs), M.build_mpo(m),
+                      SweepParams(1, ramp[0], ramp[0], 1e-8, lp, verbose=False))
+    t0 = time.perf_counter()
+    for Dr in ramp:
+        drv.p = SweepParams(1, Dr, Dr, 1e-8, lp, verbose=False)
+        drv.sweep()
+    ramp_s = time.perf_counter() - t0
+    ramp_e = drv.stats.log[-1]["e0"]
+    saved = (list(drv.mps), list(drv.lenvs), list(drv.renvs))   # updates replace tensors, never mutate them
+    out = {"model": f"Hubbard {lx}x4 cylinder, U=8, t=1, {n - n // per_hole} electrons, Sz=0",
+           "ramp": list(ramp), "ramp_s": ramp_s, "ramp_energy": ramp_e, "D": D, "timed_sweeps": nsweeps}
+    for prec in ("fp64", split):
+        drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
+        drv.p = SweepParams(1, D, D, 1e-8, lp, precision=prec, verbose=False)
+        st = drv.stats
+        for k in ("lanczos_s", "svd_s", "env_s", "contract_s", "plan_s", "matvecs", "algo_flops"):
+            setattr(st, k, 0)
+        secs, es = [], []
+        for _ in range(nsweeps):
+            es.append(drv.sweep())
+            secs.append(drv.stats.sweep_seconds[-1])
+        torch.cuda.synchronize()
+        out[prec] = {"sweep_s": secs, "energy": es[-1], "sweep_energies": es, "lanczos_s": st.lanczos_s, "svd_s": st.svd_s,
+                     "env_s": st.env_s, "contract_s": st.contract_s, "plan_s": st.plan_s, "matvecs": st.matvecs,
+                     "lanczos_tflops": st.algo_flops / max(st.lanczos_s, 1e-9) / 1e12}
+    e64, es = out["fp64"]["energy"], out[split]["energy"]
+    out["split"] = split
+    out["speedup"] = sum(out["fp64"]["sweep_s"]) / sum(out[split]["sweep_s"])
+    out["rel_dE"] = abs(es - e64) / abs(e64)
+    return out
+
+
+def time_matvec(H, x, y, steps, warmup):
+    H.ctx.synchronize()
+    for _ in range(warmup):
+        H.matvec_into(x.data, y.data, sync=False)
+    H.ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        H.matvec_into(x.data, y.data, sync=False)
+    H.ctx.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
 def cpu_baseline(eff, psi, budget_s=15.0):
     """Oracle numpy matvec on the host cores (bounded to ~budget_s seconds)."""
     from oracle.lanczos import eff_ham_mul_two_site_state, two_site_matvec_flops
@@ -91,6 +163,8 @@ def main():
     ap.add_argument("--D", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--split", default="bf16x6", help="reduced-precision mode compared with FP64")
+    ap.add_argument("--no-sweep", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the matvec is not sharded across GPUs in this build "
@@ -115,15 +189,7 @@ def main():
     sflops = H.stage_flops()
     s_algo, s_bytes = H.stage_work()
     torch.cuda.synchronize()
-    for _ in range(a.warmup):
-        H.matvec_into(x.data, y.data, sync=False)
-    H.ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        H.matvec_into(x.data, y.data, sync=False)
-    H.ctx.synchronize()
-    el = time.perf_counter() - t0
-    ms = el / a.steps * 1e3
+    ms = time_matvec(H, x, y, a.steps, a.warmup)
     # per-stage kernel times (HIP events on the plan's stream), averaged
     H.enable_timing(True)
     acc = [0.0, 0.0, 0.0]
@@ -166,6 +232,26 @@ def main():
                    "stage_algo_gflop": [f / 1e9 for f in s_algo], "stage_algo_mb": [b / 1e6 for b in s_bytes]},
         "roofline": roof,
     }
+    # the same matvec with the split-precision sector GEMMs
+    if a.split != a.precision:
+        H.set_precision(a.split)
+        ms_split = time_matvec(H, x, y, a.steps, a.warmup)
+        H.enable_timing(True)
+        acc = [0.0, 0.0, 0.0]
+        for _ in range(nt):
+            H.matvec_into(x.data, y.data, sync=True)
+            acc = [p + q for p, q in zip(acc, H.stage_ms())]
+        H.enable_timing(False)
+        H.set_precision(a.precision)
+        mult = {"fp32": 1, "bf16x6": 6, "bf16x3": 3}.get(a.split, 1)
+        out["split"] = {"precision": a.split, "value": algo / (ms_split * 1e-3) / 1e12, "unit": "TFLOP/s",
+                        "ms_per_step": ms_split, "speedup_vs_fp64": ms / ms_split,
+                        "stage_ms": [v / nt for v in acc],
+                        "issued_mfma_tflops": execf * mult / (ms_split * 1e-3) / 1e12,
+                        "note": "algorithmic FLOPs per second; issued = executed FLOPs x products per term"}
+    H.close()
+    if not a.no_sweep:
+        out["sweep"] = hubbard_sweep_comparison(split=a.split)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(eff, psi)
     print(json.dumps(out), flush=True)

@@ -6,17 +6,18 @@ import numpy as np
 import pytest
 import torch
 
-from oracle.bsten import Contract
-from oracle.dmrg import UpdateSiteLenvs, UpdateSiteRenvs, init_envs, random_mps as o_random_mps
+from oracle.bsten import BSTensor, Contract
+from oracle.dmrg import UpdateSiteLenvs, UpdateSiteRenvs, block_svd as o_block_svd, init_envs, random_mps as o_random_mps
 from oracle.models import build_mpo as o_build_mpo, heisenberg_chain as o_heis, hubbard_cylinder as o_hub
 from tests.convert import to_device, to_oracle
+from tests.workloads import synthetic_eff_ham
 
 pytestmark = pytest.mark.gpu
 
 if torch.cuda.is_available():
     from ultradmrg_amd import mpo as M
     from ultradmrg_amd.solver import LanczosParams
-    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, direct_state_mps, random_mps
+    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, block_svd, direct_state_mps, random_mps
     from ultradmrg_amd.tensor import contract, env_update
 
 
@@ -109,3 +110,65 @@ def test_dmrg_config1_heisenberg_l20():
     e = drv.run()
     assert abs(e - (-8.682473334398967)) < 1e-10 * 8.68
     print(f"[config1] E0 {e:.15f} sweep seconds {[round(s, 3) for s in drv.stats.sweep_seconds]}", flush=True)
+
+
+def _rank_deficient(psi: BSTensor, r: int, seed: int) -> BSTensor:
+    """psi with every (left charge) sector matrix replaced by a random rank <= r one."""
+    rng = np.random.default_rng(seed)
+    out = BSTensor(psi.indexes, psi.dtype, psi.div())
+    for k, b in psi.blocks.items():
+        nr, nc = b.shape[0] * b.shape[1], b.shape[2] * b.shape[3]
+        m = rng.normal(size=(nr, r)) @ rng.normal(size=(r, nc))
+        out.blocks[k] = m.reshape(b.shape).astype(psi.dtype)
+    return out
+
+
+@pytest.mark.parametrize("case", ["real", "complex", "rank_deficient"])
+def test_block_svd_parity(case):
+    """GPU block SVD (batched Gram eigen-decompositions) vs the oracle's numpy SVD
+    with the same truncation rule: same kept sectors and D, discarded weight and
+    singular values to 1e-12 (of the largest), reconstruction U S V^H equal to
+    the oracle's to 1e-11 relative, orthonormal U and V^H (1e-11).  The
+    rank-deficient case keeps 62 zero singular values, which exercises the
+    completion of their singular vectors to an orthonormal set."""
+    print(f"[case] block svd {case}", flush=True)
+    eff, psi = synthetic_eff_ham("hubbard", 160, seed=4, nsect_half=3)
+    if case == "complex":
+        rng = np.random.default_rng(6)
+        pc = BSTensor(psi.indexes, np.complex128, psi.div())
+        pc.blocks = {k: b + 1j * rng.normal(size=b.shape) for k, b in psi.blocks.items()}
+        psi = pc
+    trunc, Dmin, Dmax = 1e-8, 60, 100
+    if case == "rank_deficient":
+        psi = _rank_deficient(psi, 1, 5)     # numerical rank 238 of 390
+        trunc, Dmin, Dmax = 0.0, 300, 300
+    ldiv = tuple(0 for _ in psi.div())
+    ref = o_block_svd(psi, 2, ldiv, trunc, Dmin, Dmax)
+    out = block_svd(to_device(psi), 2, ldiv, trunc, Dmin, Dmax)
+    assert out.D == ref.D
+    assert abs(out.trunc_err - ref.trunc_err) <= 1e-12
+    so = out.s.blocks()
+    smax = max(float(np.abs(v).max()) for v in ref.s.blocks.values())
+    if case != "rank_deficient":
+        assert sorted(so) == sorted(ref.s.blocks)
+        for k, v in ref.s.blocks.items():
+            assert np.abs(np.diag(so[k]) - np.diag(v)).max() <= 1e-12 * smax
+    else:   # kept zero singular values: their sector is decided by rounding noise
+        g = np.sort(np.concatenate([np.diag(b) for b in so.values()]))
+        r = np.sort(np.concatenate([np.diag(b) for b in ref.s.blocks.values()]))
+        assert np.abs(g - r).max() <= 1e-12 * smax
+    # reconstruction
+    rec = contract(contract(out.u, out.s, 1), out.vt, 1)
+    oref = Contract(Contract(ref.u, ref.s, ([2], [0])), ref.vt, ([2], [0]))
+    got = to_oracle(rec, oref)
+    num = sum(float(np.sum(np.abs(got.blocks[k] - oref.blocks[k]) ** 2)) for k in oref.blocks)
+    den = sum(float(np.sum(np.abs(oref.blocks[k]) ** 2)) for k in oref.blocks)
+    assert (num / den) ** 0.5 < 1e-11
+    # orthonormality per kept sector
+    ub, vb = out.u.blocks(), out.vt.blocks()
+    for mi in range(len(out.u.axes[-1].degs)):
+        U = np.concatenate([b.reshape(-1, b.shape[-1]) for k, b in sorted(ub.items()) if k[-1] == mi], axis=0)
+        V = np.concatenate([b.reshape(b.shape[0], -1) for k, b in sorted(vb.items()) if k[0] == mi], axis=1)
+        k = U.shape[1]
+        assert np.abs(U.conj().T @ U - np.eye(k)).max() < 1e-11
+        assert np.abs(V @ V.conj().T - np.eye(k)).max() < 1e-11

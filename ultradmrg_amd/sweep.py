@@ -63,54 +63,98 @@ class SVDResult:
     entropy: float
 
 
-def _sector_svds(mats) -> dict:
-    """Thin SVDs of the per-sector matrices through the eigen-decomposition of
-    the Gram matrix on each matrix's smaller side, batched on the GPU by size
-    bucket (rocSOLVER's batched symmetric eigensolver; a per-matrix loop is
-    latency bound for the ~100 small sectors of a D=2000 state).  Padded rows
-    and columns carry eigenvalue -1 so they separate from the non-negative
-    spectrum.  Returns q -> (U, S, Vh, rks, cks, ro, co, g) with S descending;
-    singular vectors of (near-)zero singular values are completed to an
-    orthonormal set."""
-    buckets: Dict[int, list] = {}
-    for q, (M, *_rest) in mats.items():
-        n = min(M.shape)
-        b = 128
-        while b < n:
-            b *= 2
-        buckets.setdefault(b, []).append(q)
-    out = {}
-    for b, qs in buckets.items():
-        dev, dt = mats[qs[0]][0].device, mats[qs[0]][0].dtype
-        G = torch.zeros((len(qs), b, b), dtype=dt, device=dev)
-        for j, q in enumerate(qs):
-            M = mats[q][0]
-            n = min(M.shape)
-            G[j, :n, :n] = M @ M.mT.conj() if M.shape[0] <= M.shape[1] else M.mT.conj() @ M
-            if n < b:
-                G[j, n:, n:] = -torch.eye(b - n, dtype=dt, device=dev)
-        w, V = torch.linalg.eigh(G)
-        for j, q in enumerate(qs):
-            M, rks, cks, ro, co, g = mats[q]
-            n = min(M.shape)
-            lam = w[j, b - n:].flip(0).clamp_min(0.0)           # descending
-            vec = V[j, :n, b - n:].flip(1)
-            S = lam.sqrt().to(torch.float64)
-            tiny = S <= 1e-14 * max(float(S[0]), 1e-300)
-            Sinv = torch.where(tiny, torch.zeros_like(S), 1.0 / torch.where(tiny, torch.ones_like(S), S))
-            if M.shape[0] <= M.shape[1]:
-                U = vec
-                Vh = (U.mT.conj() @ M) * Sinv[:, None].to(dt)
-                if bool(tiny.any()):
-                    Vh = _complete_rows(Vh, tiny)
-            else:
-                Vm = vec
-                U = (M @ Vm) * Sinv[None, :].to(dt)
-                if bool(tiny.any()):
-                    U = _complete_rows(U.mT, tiny).mT
-                Vh = Vm.mT.conj()
-            out[q] = (U, S, Vh, rks, cks, ro, co, g)
-    return out
+# padded sizes of the batched eigen-decompositions.  rocSOLVER's tridiagonal
+# reduction is latency bound (hundreds of small launches per call), so few
+# buckets with coarse steps beat tight padding: at D=2000 (sectors up to ~750)
+# (256, 768) costs 37 ms per SVD against 64 ms for 1.5x steps from 64.
+_BUCKETS = (256, 768, 2048, 4096, 8192)
+_SV_NOISE = 1e-6   # singular values at or below this fraction of their sector's largest count as zero
+
+
+def _bucket(n: int) -> int:
+    for b in _BUCKETS:
+        if n <= b:
+            return b
+    return -(-n // 4096) * 4096
+
+
+class _SvdStructure:
+    """Sector grouping of a tensor's block structure for a block SVD, and the
+    flat gather index that places every element of the tensor into padded,
+    per-size-bucket batches of sector matrices.  Each sector matrix is stored
+    with its shorter side first (transposed when it has more rows than
+    columns), so its Gram matrix on that side is one batched product."""
+
+    def __init__(self, t: BlockTensor, ldims: int, ldiv):
+        left, right = t.axes[:ldims], t.axes[ldims:]
+        groups: Dict[tuple, dict] = {}
+        for i, c in enumerate(t.coords):
+            q = tuple(ldiv)
+            for ax, s in zip(left, c[:ldims]):
+                sgn = -1 if ax.dir == OUT else 1
+                q = tuple(a + sgn * b for a, b in zip(q, ax.qns[s]))
+            g = groups.setdefault(q, {"rows": {}, "cols": {}, "blocks": []})
+            lk, rk = c[:ldims], c[ldims:]
+            g["rows"].setdefault(lk, int(np.prod([ax.degs[s] for ax, s in zip(left, lk)])))
+            g["cols"].setdefault(rk, int(np.prod([ax.degs[s] for ax, s in zip(right, rk)])))
+            g["blocks"].append((i, lk, rk))
+        self.qs = sorted(groups)
+        self.sect = []
+        for q in self.qs:
+            g = groups[q]
+            rks, cks = sorted(g["rows"]), sorted(g["cols"])
+            ro, co, acc = {}, {}, 0
+            for k in rks:
+                ro[k] = acc
+                acc += g["rows"][k]
+            R = acc
+            acc = 0
+            for k in cks:
+                co[k] = acc
+                acc += g["cols"][k]
+            Cn = acc
+            self.sect.append(dict(q=q, rows=g["rows"], cols=g["cols"], rks=rks, cks=cks, ro=ro, co=co, R=R, C=Cn,
+                                  tr=R > Cn, n=min(R, Cn), m=max(R, Cn), blocks=g["blocks"]))
+        # buckets of sectors by padded short side; slot j of bucket b
+        self.buckets: Dict[int, List[int]] = {}
+        for si, sc in enumerate(self.sect):
+            self.buckets.setdefault(_bucket(sc["n"]), []).append(si)
+        self.mpad = {b: max(16, -(-max(self.sect[si]["m"] for si in sis) // 16) * 16)
+                     for b, sis in self.buckets.items()}
+        # gather: element e of t.data -> flat position in the concatenated bucket batches
+        dst = np.full(t.data.numel(), -1, dtype=np.int64)   # -1: element of no block
+        base = 0
+        self.bucket_base = {}
+        for b, sis in self.buckets.items():
+            mp = self.mpad[b]
+            self.bucket_base[b] = base
+            for j, si in enumerate(sis):
+                sc = self.sect[si]
+                sbase = base + j * b * mp
+                for (i, lk, rk) in sc["blocks"]:
+                    nr, nc = sc["rows"][lk], sc["cols"][rk]
+                    rr = sc["ro"][lk] + np.arange(nr)[:, None]
+                    cc = sc["co"][rk] + np.arange(nc)[None, :]
+                    pos = sbase + (cc * mp + rr if sc["tr"] else rr * mp + cc)
+                    off = t.offsets[i]
+                    dst[off:off + nr * nc] = pos.reshape(-1)
+            base += len(sis) * b * mp
+        self.total = base
+        dst[dst < 0] = base          # one trash slot past the batches
+        self.dst = torch.from_numpy(dst).to(t.data.device)
+
+
+_SVD_CACHE: Dict[tuple, _SvdStructure] = {}
+
+
+def _svd_structure(t: BlockTensor, ldims: int, ldiv) -> _SvdStructure:
+    key = (tuple(t.axes), tuple(t.coords), tuple(t.offsets), ldims, tuple(ldiv), t.data.numel())
+    st = _SVD_CACHE.get(key)
+    if st is None:
+        if len(_SVD_CACHE) > 256:
+            _SVD_CACHE.clear()
+        st = _SVD_CACHE[key] = _SvdStructure(t, ldims, ldiv)
+    return st
 
 
 def _complete_rows(A: torch.Tensor, bad: torch.Tensor) -> torch.Tensor:
@@ -126,44 +170,51 @@ def _complete_rows(A: torch.Tensor, bad: torch.Tensor) -> torch.Tensor:
 
 
 def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dmax: int) -> SVDResult:
+    """Block SVD with truncation, through FP64 eigen-decompositions of each
+    sector matrix's Gram matrix on its shorter side, batched by size bucket.
+
+    Truncation (this build's rule): pool the singular values of all sectors,
+    keep the smallest D whose discarded weight sum(s^2)/total <= trunc_err,
+    clamped to [Dmin, Dmax].  The Gram route resolves singular values to an
+    absolute ~sqrt(n eps) of the sector's largest, so values at or below 1e-6
+    of it (Schmidt weight <= 1e-12 relative) are returned as zero, and the
+    singular vectors of kept zero values are completed to an orthonormal set."""
     left, right = t.axes[:ldims], t.axes[ldims:]
-    ldiv = tuple(ldiv)
     dev, dt = t.data.device, t.data.dtype
-    groups: Dict[tuple, dict] = {}
-    for i, c in enumerate(t.coords):
-        q = ldiv
-        for ax, s in zip(left, c[:ldims]):
-            sgn = -1 if ax.dir == OUT else 1
-            q = tuple(a + sgn * b for a, b in zip(q, ax.qns[s]))
-        g = groups.setdefault(q, {"rows": {}, "cols": {}, "blocks": []})
-        lk, rk = c[:ldims], c[ldims:]
-        g["rows"].setdefault(lk, int(np.prod([ax.degs[s] for ax, s in zip(left, lk)])))
-        g["cols"].setdefault(rk, int(np.prod([ax.degs[s] for ax, s in zip(right, rk)])))
-        g["blocks"].append((i, lk, rk))
-    mats = {}
-    for q in sorted(groups):
-        g = groups[q]
-        rks, cks = sorted(g["rows"]), sorted(g["cols"])
-        ro, co = {}, {}
-        acc = 0
-        for k in rks:
-            ro[k] = acc
-            acc += g["rows"][k]
-        R = acc
-        acc = 0
-        for k in cks:
-            co[k] = acc
-            acc += g["cols"][k]
-        Cn = acc
-        M = torch.zeros((R, Cn), dtype=dt, device=dev)
-        for i, lk, rk in g["blocks"]:
-            M[ro[lk]:ro[lk] + g["rows"][lk], co[rk]:co[rk] + g["cols"][rk]] = \
-                block_view(t, i).reshape(g["rows"][lk], g["cols"][rk])
-        mats[q] = (M, rks, cks, ro, co, g)
-    dec = _sector_svds(mats)
-    svals = [dec[q][1] for q in sorted(groups)]
-    sv = torch.cat(svals).double().cpu().numpy()
-    owner = np.concatenate([[qi] * len(dec[q][1]) for qi, q in enumerate(sorted(groups))])
+    st = _svd_structure(t, ldims, ldiv)
+    flat = torch.zeros(st.total + 1, dtype=dt, device=dev)
+    flat[st.dst] = t.data
+    per = {}   # bucket -> (E [B, b, b] descending eigenvectors, S [B, b], P = E^T X / S [B, b, mpad])
+    svals = []
+    for b, sis in st.buckets.items():
+        mp, B = st.mpad[b], len(sis)
+        X = flat[st.bucket_base[b]:st.bucket_base[b] + B * b * mp].view(B, b, mp)
+        G = X @ X.mT.conj()
+        ns = torch.tensor([st.sect[si]["n"] for si in sis], device=dev)
+        pad = torch.arange(b, device=dev)[None, :] >= ns[:, None]
+        G.diagonal(dim1=1, dim2=2)[pad] = -1.0     # padded rows separate below the spectrum
+        w, V = torch.linalg.eigh(G)
+        lam = w.flip(-1).clamp_min(0.0)
+        E = V.flip(-1)
+        S = lam.sqrt()
+        # Gram eigenvalues carry an absolute error ~ n eps lambda_max, i.e. singular
+        # values below ~1e-7 of the sector's largest are noise: they are set to
+        # zero and their singular vectors completed to an orthonormal set below
+        S = torch.where(S > _SV_NOISE * S[:, :1], S, torch.zeros_like(S))
+        Sinv = torch.where(S > 0, 1.0 / torch.where(S > 0, S, torch.ones_like(S)), torch.zeros_like(S))
+        P = (E.mT.conj() @ X) * Sinv[..., None].to(dt)
+        per[b] = (E, S, P)
+        svals.append(S)
+    # one host transfer for the truncation decision
+    sv_all = [s.double().cpu().numpy() for s in svals]
+    sv, owner = [], []
+    for (b, sis), arr in zip(st.buckets.items(), sv_all):
+        for j, si in enumerate(sis):
+            n = st.sect[si]["n"]
+            sv.append(arr[j, :n])
+            owner.append(np.full(n, si))
+    sv = np.concatenate(sv)
+    owner = np.concatenate(owner)
     order = np.argsort(-sv, kind="stable")
     sq = sv[order] ** 2
     total = float(sq.sum())
@@ -173,35 +224,50 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         if tail[k] / total <= trunc_err:
             D = k
             break
-    D = min(max(D, min(Dmin, len(sv))), Dmax, len(sv))
-    D = max(D, 1)
+    D = max(min(max(D, min(Dmin, len(sv))), Dmax, len(sv)), 1)
     err = float(tail[D] / total)
-    qs_sorted = sorted(groups)
-    keep = {}
-    for idx in order[:D]:
-        q = qs_sorted[owner[idx]]
-        keep[q] = keep.get(q, 0) + 1
-    kept_qs = [q for q in qs_sorted if q in keep]
-    mid_out = Axis.make([(q, keep[q]) for q in kept_qs], OUT)
+    keep = np.bincount(owner[order[:D]], minlength=len(st.sect))
+    smax = float(sv[order[0]]) if len(sv) else 0.0
+    kept_si = [si for si in range(len(st.sect)) if keep[si] > 0]
+    mid_out = Axis.make([(st.sect[si]["q"], int(keep[si])) for si in kept_si], OUT)
     mid_in = mid_out.inverse()
+    slot = {}
+    for b, sis in st.buckets.items():
+        for j, si in enumerate(sis):
+            slot[si] = (b, j)
     up, sp, vp = {}, {}, {}
-    for mi, q in enumerate(kept_qs):
-        U, S, Vh, rks, cks, ro, co, g = dec[q]
-        k = keep[q]
-        sp[(mi, mi)] = torch.diag(S[:k]).to(dt)
-        for lk in rks:
+    for mi, si in enumerate(kept_si):
+        sc, k = st.sect[si], int(keep[si])
+        b, j = slot[si]
+        E, S, P = per[b]
+        n, m = sc["n"], sc["m"]
+        Sk = S[j, :k]
+        if sc["tr"]:     # X = M^T: E = conj(V), so V^H = E^T and U = M conj(E) / S = P^T
+            U = P[j, :k, :m].mT
+            Vh = E[j, :n, :k].mT
+        else:
+            U = E[j, :n, :k]
+            Vh = P[j, :k, :m]
+        if float(Sk[-1]) == 0.0:
+            bad = Sk == 0
+            if sc["tr"]:
+                U = _complete_rows(U.mT.contiguous(), bad).mT
+            else:
+                Vh = _complete_rows(Vh.contiguous(), bad)
+        sp[(mi, mi)] = torch.diag(Sk).to(dt)
+        for lk in sc["rks"]:
             shp = tuple(ax.degs[s] for ax, s in zip(left, lk)) + (k,)
-            up[lk + (mi,)] = U[ro[lk]:ro[lk] + g["rows"][lk], :k].reshape(shp)
-        for rk in cks:
+            up[lk + (mi,)] = U[sc["ro"][lk]:sc["ro"][lk] + sc["rows"][lk], :].reshape(shp)
+        for rk in sc["cks"]:
             shp = (k,) + tuple(ax.degs[s] for ax, s in zip(right, rk))
-            vp[(mi,) + rk] = Vh[:k, co[rk]:co[rk] + g["cols"][rk]].reshape(shp)
+            vp[(mi,) + rk] = Vh[:, sc["co"][rk]:sc["co"][rk] + sc["cols"][rk]].reshape(shp)
     u = from_pieces(list(left) + [mid_out], up, dt, dev)
     s = from_pieces([mid_in, mid_out], sp, dt, dev)
     vt = from_pieces([mid_in] + list(right), vp, dt, dev)
-    p = np.sort(sv)[::-1][:D] ** 2
-    p = p / p.sum()
-    p = p[p > 0]
-    ee = float(-(p * np.log(p)).sum())
+    pr = np.sort(sv)[::-1][:D] ** 2
+    pr = pr / pr.sum()
+    pr = pr[pr > 0]
+    ee = float(-(pr * np.log(pr)).sum())
     return SVDResult(u, s, vt, err, D, ee)
 
 
