@@ -119,6 +119,27 @@ int qlg_plan_two_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *Wl,
 int qlg_plan_one_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *W,
                       const qlg_bsten *R, const qlg_bsten *psi,
                       int precision, qlg_plan *out);
+/* DMRG-form (super-block) operator: replaces the SuperBlockHamiltonianTerms
+ * argument of LanczosSolver(SuperBlockHamiltonianTerms<TenT>&, TenT*, const
+ * LanczosParams&), include/qlmps/algorithm/dmrg/lanczos_dmrg_solver_impl.h:61-66,
+ * whose matvec (:199-283) is
+ *   H psi = sum_j [sum_i L_i (x) W_l(i, j)] psi [sum_k W_r(j, k) (x) R_k],
+ * with the term groups of DMRGExecutor::SetEffectiveHamiltonianTerms_
+ * (include/qlmps/algorithm/dmrg/dmrg_impl.h:319-341).  Group j holds
+ * nleft[j] pairs {L_i, W_l(i, j)} and nright[j] pairs {W_r(j, k), R_k}; the
+ * groups' pairs are stored back to back in left[] and right[].  Block
+ * operators are rank 2, L_i[a (OUT), a' (IN)] and R_k[b (IN), b' (OUT)]; site
+ * operators are W[s_in (IN), s_out (OUT)].  A block operator used by several
+ * terms is passed as the same descriptor pointer (the reference shares it by
+ * pointer).  The plan stacks the operators into MPO form once (bond elements
+ * grouped by charge) and is then used like a two-site plan.               */
+typedef struct {
+  const qlg_bsten *op0, *op1;  /* {L_i, W_l(i,j)} or {W_r(j,k), R_k} */
+} qlg_term;
+int qlg_plan_super_block(qlg_ctx ctx, int64_t ngroups, const int64_t *nleft,
+                         const qlg_term *left, const int64_t *nright,
+                         const qlg_term *right, const qlg_bsten *psi,
+                         int precision, qlg_plan *out);
 int qlg_plan_destroy(qlg_plan plan);
 int qlg_plan_set_precision(qlg_plan plan, int precision);
 /* Algorithmic FLOPs of one matvec: sum of 2*M*N*K (8*M*N*K complex) over the

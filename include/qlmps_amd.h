@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -283,6 +284,20 @@ struct ExpmvRes {
 };
 
 // A planned effective Hamiltonian: sector pairing done once.
+// DMRG-form term groups, include/qlmps/algorithm/dmrg/dmrg.h:16-45 (same
+// names and nesting): group j pairs {L_i, W_l(i, j)} with {W_r(j, k), R_k}.
+template <typename TenT>
+using BlockSiteHamiltonianTerm = std::array<TenT *, 2>;
+template <typename TenT>
+using BlockSiteHamiltonianTermGroup = std::vector<BlockSiteHamiltonianTerm<TenT>>;
+template <typename TenT>
+using SiteBlockHamiltonianTerm = std::array<TenT *, 2>;
+template <typename TenT>
+using SiteBlockHamiltonianTermGroup = std::vector<SiteBlockHamiltonianTerm<TenT>>;
+template <typename TenT>
+using SuperBlockHamiltonianTerms =
+    std::vector<std::pair<BlockSiteHamiltonianTermGroup<TenT>, SiteBlockHamiltonianTermGroup<TenT>>>;
+
 template <typename Elem>
 class EffectiveHamiltonian {
  public:
@@ -301,6 +316,30 @@ class EffectiveHamiltonian {
     } else {
       throw std::runtime_error("effective Hamiltonian must be {L, Wl, Wr, R} or {L, W, R}");
     }
+    ReadLayout(state);
+  }
+  // DMRG form: the term groups of SuperBlockHamiltonianTerms (see below).
+  EffectiveHamiltonian(const SuperBlockHamiltonianTerms<TenT> &terms, const TenT &state,
+                       int precision = QLG_PREC_F64)
+      : ctx_(&state.ctx()) {
+    const qlg_ctx c = ctx_->get();
+    std::vector<int64_t> nl, nr;
+    std::vector<qlg_term> lt, rt;
+    for (const auto &g : terms) {
+      nl.push_back((int64_t)g.first.size());
+      nr.push_back((int64_t)g.second.size());
+      for (const auto &t : g.first) lt.push_back(qlg_term{&t[0]->Descriptor(), &t[1]->Descriptor()});
+      for (const auto &t : g.second) rt.push_back(qlg_term{&t[0]->Descriptor(), &t[1]->Descriptor()});
+    }
+    Check(qlg_plan_super_block(c, (int64_t)terms.size(), nl.data(), lt.data(), nr.data(), rt.data(),
+                               &state.Descriptor(), precision, &plan_),
+          c);
+    ReadLayout(state);
+  }
+
+ private:
+  void ReadLayout(const TenT &state) {
+    const qlg_ctx c = ctx_->get();
     int64_t nb = 0, sz = 0;
     Check(qlg_plan_vector_layout(plan_, &nb, &sz, nullptr, nullptr), c);
     std::vector<int32_t> flat(nb * state.Rank());
@@ -311,6 +350,8 @@ class EffectiveHamiltonian {
     size_ = (size_t)sz;
     idx_ = state.GetIndexes();
   }
+
+ public:
   ~EffectiveHamiltonian() { qlg_plan_destroy(plan_); }
   EffectiveHamiltonian(const EffectiveHamiltonian &) = delete;
   EffectiveHamiltonian &operator=(const EffectiveHamiltonian &) = delete;
@@ -391,6 +432,27 @@ LanczosRes<TenT> LanczosSolver(const std::vector<TenT *> &rpeff_ham, TenT *pinit
   EffectiveHamiltonian<typename std::remove_const<Elem>::type> H(rpeff_ham, *pinit_state);
   TenT *work = H.Adopt(*pinit_state);
   delete pinit_state;  // the solver consumes the initial state
+  qlg_lanczos_params prm{params.error, (int64_t)params.max_iterations};
+  qlg_lanczos_result res{};
+  const int rc = qlg_lanczos_gs(H.plan(), work->DeviceData(), &prm, &res);
+  if (rc != QLG_OK) {
+    const qlg_ctx c = work->ctx().get();
+    delete work;
+    Check(rc, c);
+  }
+  return LanczosRes<TenT>{(size_t)res.iters, res.gs_eng, work};
+}
+
+// DMRG overload, include/qlmps/algorithm/dmrg/lanczos_dmrg_solver_impl.h:61-196:
+// the same recurrence on the super-block operator; the initial state is
+// consumed and gs_vec is new-ed for the caller.
+template <typename TenT>
+LanczosRes<TenT> LanczosSolver(SuperBlockHamiltonianTerms<TenT> &terms, TenT *pinit_state,
+                               const LanczosParams &params) {
+  using Elem = typename std::remove_pointer<decltype(pinit_state->DeviceData())>::type;
+  EffectiveHamiltonian<typename std::remove_const<Elem>::type> H(terms, *pinit_state);
+  TenT *work = H.Adopt(*pinit_state);
+  delete pinit_state;
   qlg_lanczos_params prm{params.error, (int64_t)params.max_iterations};
   qlg_lanczos_result res{};
   const int rc = qlg_lanczos_gs(H.plan(), work->DeviceData(), &prm, &res);

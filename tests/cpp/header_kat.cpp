@@ -66,6 +66,41 @@ int main(int argc, char **argv) {
     std::printf("E %.15f ITERS %zu NORM %.15f\n", res.gs_eng, res.iters, res.gs_vec->Norm());
     delete res.gs_vec;
   }
+  // the same operator in the DMRG (super-block) form of lanczos_dmrg_solver_impl.h:61-66:
+  // one term group per MPO bond value k, {(L_k, m)} with {(m, R_k)}, L_k[a, a'] = lblock(a', k, a)
+  // = M[a'][a] and R_k[b, b'] = rblock(b', k, b) = M[b][b']
+  {
+    std::vector<DTen> Lk, Rk;
+    for (size_t k = 0; k < dh; ++k) {
+      Lk.emplace_back(std::vector<Index>{Dout, Din});
+      Rk.emplace_back(std::vector<Index>{Din, Dout});
+    }
+    std::vector<double> Mt(D * D);
+    for (size_t a = 0; a < D; ++a)
+      for (size_t b = 0; b < D; ++b) Mt[a * D + b] = M[b * D + a];
+    for (size_t k = 0; k < dh; ++k) {
+      Lk[k].SetBlock({0, 0}, Mt);
+      Lk[k].Upload();
+      Rk[k].SetBlock({0, 0}, M);
+      Rk[k].Upload();
+    }
+    DTen msite({din, dout});
+    msite.SetBlock({0, 0}, m);
+    msite.Upload();
+    SuperBlockHamiltonianTerms<DTen> terms;
+    for (size_t k = 0; k < dh; ++k)
+      terms.emplace_back(BlockSiteHamiltonianTermGroup<DTen>{{&Lk[k], &msite}},
+                         SiteBlockHamiltonianTermGroup<DTen>{{&msite, &Rk[k]}});
+    auto *init = new DTen({Din, dout, dout, Dout});
+    std::vector<double> v(D * d * d * D);
+    srand(12345);
+    for (auto &x : v) x = Rand() - 0.5;
+    init->SetBlock({0, 0, 0, 0}, v);
+    init->Upload();
+    auto res = LanczosSolver<DTen>(terms, init, LanczosParams(1.0E-9));
+    std::printf("SB %.15f ITERS %zu NORM %.15f\n", res.gs_eng, res.iters, res.gs_vec->Norm());
+    delete res.gs_vec;
+  }
   // tridiagonal helpers with the reference signatures
   double e;
   double *vec = nullptr;

@@ -24,6 +24,8 @@ def test_header_layer_lanczos_kat():
     lines = [l.split() for l in r.stdout.splitlines()]
     runs = [(float(l[1]), int(l[3]), float(l[5])) for l in lines if l[0] == "E"]
     assert len(runs) == 2
+    sb = [(float(l[1]), int(l[3])) for l in lines if l[0] == "SB"]
+    assert len(sb) == 1
     L, Wl, Wr, R, (Din, dout, Dout) = refkat.kat_eff_ham(20, 2, 2)
     t = np.einsum("xwa,wsty->xatsy", L.to_dense(), Wl.to_dense())
     t = np.einsum("xatsy,yuvz->xtvzasu", t, Wr.to_dense())
@@ -40,6 +42,9 @@ def test_header_layer_lanczos_kat():
         assert iters == o.iters
         assert abs(e - o.gs_eng) <= 1e-12 * abs(o.gs_eng)
         assert abs(nrm - 1.0) < 1e-12
+    # DMRG-form overload on the same operator: same energy and iteration count as the first case
+    assert abs(sb[0][0] - w0) < 1e-8
+    assert abs(sb[0][0] - runs[0][0]) <= 1e-12 * abs(w0) and sb[0][1] == runs[0][1]
     tri = [l for l in lines if l[0] == "TRIDIAG"][0]
     e3, v3 = TridiagGsSolver([2.0, 1.0, 3.0], [0.5, 0.25], 3, "V")
     assert abs(float(tri[1]) - e3) < 1e-14 and abs(abs(float(tri[2])) - abs(v3[0])) < 1e-14

@@ -26,7 +26,7 @@ ERRORS = {1: "QLG_ERR_INVALID", 2: "QLG_ERR_HIP", 3: "QLG_ERR_ALLOC", 4: "QLG_ER
 EXPORTS = [
     "qlg_ctx_create", "qlg_ctx_destroy", "qlg_last_error", "qlg_ctx_synchronize",
     "qlg_device_alloc", "qlg_device_free", "qlg_memcpy_h2d", "qlg_memcpy_d2h", "qlg_memcpy_d2d",
-    "qlg_plan_two_site", "qlg_plan_one_site", "qlg_plan_destroy", "qlg_plan_set_precision",
+    "qlg_plan_two_site", "qlg_plan_one_site", "qlg_plan_super_block", "qlg_plan_destroy", "qlg_plan_set_precision",
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
     "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
     "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
@@ -48,6 +48,10 @@ class Bsten(C.Structure):
         ("nblocks", C.c_int64), ("blk_coords", C.POINTER(C.c_int32)),
         ("blk_offset", C.POINTER(C.c_int64)), ("dtype", C.c_int32), ("data", C.c_void_p),
     ]
+
+
+class Term(C.Structure):
+    _fields_ = [("op0", C.POINTER(Bsten)), ("op1", C.POINTER(Bsten))]
 
 
 class LanczosParams(C.Structure):
@@ -81,6 +85,8 @@ def lib():
         "qlg_memcpy_d2d": (C.c_int, [vp, vp, vp, i64]),
         "qlg_plan_two_site": (C.c_int, [vp] + [C.POINTER(Bsten)] * 5 + [C.c_int, C.POINTER(vp)]),
         "qlg_plan_one_site": (C.c_int, [vp] + [C.POINTER(Bsten)] * 4 + [C.c_int, C.POINTER(vp)]),
+        "qlg_plan_super_block": (C.c_int, [vp, i64, C.POINTER(i64), C.POINTER(Term), C.POINTER(i64),
+                                           C.POINTER(Term), C.POINTER(Bsten), C.c_int, C.POINTER(vp)]),
         "qlg_plan_destroy": (C.c_int, [vp]),
         "qlg_plan_set_precision": (C.c_int, [vp, C.c_int]),
         "qlg_plan_flops": (C.c_int, [vp, dp, dp]),

@@ -157,6 +157,19 @@ int qlg_plan_one_site(qlg_ctx ctx, const qlg_bsten *L, const qlg_bsten *W, const
   return make_plan(ctx, 1, L, Ws, R, psi, precision, out);
 }
 
+int qlg_plan_super_block(qlg_ctx ctx, int64_t ngroups, const int64_t *nleft, const qlg_term *left,
+                         const int64_t *nright, const qlg_term *right, const qlg_bsten *psi, int precision,
+                         qlg_plan *out) {
+  if (!ctx || !out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    check_precision(precision, psi ? psi->dtype : QLG_F64);
+    auto h = std::make_unique<qlg_plan_s>();
+    h->p.reset(build_super_block_plan(&ctx->c, ngroups, nleft, left, nright, right, psi, precision));
+    *out = h.release();
+  });
+}
+
 int qlg_plan_destroy(qlg_plan plan) {
   if (!plan) return QLG_OK;
   (void)hipSetDevice(plan->p->ctx->device);

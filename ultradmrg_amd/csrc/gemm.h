@@ -73,6 +73,13 @@ struct CopyDesc {
   int32_t rows, cols;
 };
 
+// transposed 2-D copy: dst[x * dsx + z] = src[z * sld + x] for z < nz, x < nx
+struct TransDesc {
+  int64_t src, dst;
+  int64_t sld, dsx;
+  int32_t nz, nx;
+};
+
 struct GemmTile {
   int32_t task;
   int32_t m0, n0;

@@ -202,6 +202,33 @@ __global__ __launch_bounds__(256) void block_copy_kernel(const CopyDesc *__restr
   }
 }
 
+// transposed copies (stacking block operators): one workgroup per descriptor
+template <bool CPLX>
+__global__ __launch_bounds__(256) void trans_copy_kernel(const TransDesc *__restrict__ d, const double *__restrict__ src,
+                                                         double *__restrict__ dst, int n) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const TransDesc c = d[i];
+  const int64_t tot = (int64_t)c.nz * c.nx;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) {
+    const int64_t x = e / c.nz, z = e % c.nz;   // consecutive threads write consecutive z
+    if constexpr (CPLX)
+      reinterpret_cast<double2 *>(dst)[c.dst + x * c.dsx + z] =
+          reinterpret_cast<const double2 *>(src)[c.src + z * c.sld + x];
+    else
+      dst[c.dst + x * c.dsx + z] = src[c.src + z * c.sld + x];
+  }
+}
+
+void launch_trans_copy(bool cplx, const TransDesc *d, int n, const double *src, double *dst, hipStream_t st) {
+  if (n <= 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(trans_copy_kernel<true>, dim3(n), dim3(256), 0, st, d, src, dst, n);
+  else
+    hipLaunchKernelGGL(trans_copy_kernel<false>, dim3(n), dim3(256), 0, st, d, src, dst, n);
+  QLG_HIP(hipGetLastError());
+}
+
 void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles, const ChanTask *tasks,
                       const RowDesc *rows, const void *C, const void *X, void *Y, int ntiles, hipStream_t st) {
   if (ntiles <= 0) return;

@@ -7,6 +7,7 @@
 //  stage 3  per bra sector b': out = Y_b' . R_b'^T        (grouped GEMM)
 // Tensordot and environment-update plans (ctr.cpp) reuse the grouped GEMM.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "common.h"
@@ -81,6 +82,7 @@ struct Plan {
   bool reduced_ready = false;
   void prepare_reduced();
   DevBuf vtmp_in, vtmp_out;     // staging for qlg_matvec on canonical vectors
+  std::vector<std::unique_ptr<DevBuf>> owned;   // operators the plan built itself (super-block form)
   ~Plan();
   void matvec(const double *in, double *out);        // canonical in / out
   void matvec_fused(const double *in, double *out);  // fused in / out
@@ -91,6 +93,8 @@ struct Plan {
 
 Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
                  const qlg_bsten *psi, int precision);
+Plan *build_super_block_plan(Ctx *ctx, int64_t ngroups, const int64_t *nleft, const qlg_term *left,
+                             const int64_t *nright, const qlg_term *right, const qlg_bsten *psi, int precision);
 Plan *build_tensordot_plan(Ctx *ctx, const qlg_bsten *A, const qlg_bsten *B, int k, int conj_a, int conj_b);
 Plan *build_env_plan(Ctx *ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side);
 

@@ -57,14 +57,44 @@ class ExpmvRes:
 class EffectiveHamiltonian:
     """A planned H_eff = L . W... . R acting on states with ``state``'s structure."""
 
-    def __init__(self, eff_ham: Sequence[BlockTensor], state: BlockTensor, precision: str = "fp64",
+    def __init__(self, eff_ham, state: BlockTensor, precision: str = "fp64",
                  ctx: Optional[Context] = None):
+        """``eff_ham``: [L, W_l, W_r, R] (two-site), [L, W, R] (one-site), or the
+        DMRG form's SuperBlockHamiltonianTerms: a list over the middle MPO bond
+        of pairs (block-site terms [(L_i, W_l(i, j)), ...], site-block terms
+        [(W_r(j, k), R_k), ...]) (``lanczos_dmrg_solver_impl.h:199-283``)."""
         self.ctx = ctx or default_context(state.data.device.index or 0)
         self.ops = list(eff_ham)          # keep operand buffers alive
         lib = self.ctx.lib
         h = C.c_void_p()
-        descs = [t.descriptor() for t in self.ops]
         sd = state.descriptor()
+        if self.ops and isinstance(self.ops[0], tuple):
+            self._plan_super_block(self.ops, sd, precision, h)
+        else:
+            self._plan_mpo_form(sd, precision, h)
+        self._finish(h, state)
+
+    def _plan_super_block(self, terms, sd, precision, h):
+        lib = self.ctx.lib
+        descs = {}                        # one descriptor per distinct tensor object
+
+        def dptr(t):
+            if id(t) not in descs:
+                descs[id(t)] = (t, t.descriptor())
+            return C.pointer(descs[id(t)][1])
+        nl = (C.c_int64 * len(terms))(*[len(bs) for bs, _ in terms])
+        nr = (C.c_int64 * len(terms))(*[len(sb) for _, sb in terms])
+        lt = [(L_i, W) for bs, _ in terms for L_i, W in bs]
+        rt = [(W, R_k) for _, sb in terms for W, R_k in sb]
+        left = (L.Term * max(len(lt), 1))(*[L.Term(dptr(a), dptr(b)) for a, b in lt])
+        right = (L.Term * max(len(rt), 1))(*[L.Term(dptr(a), dptr(b)) for a, b in rt])
+        torch.cuda.synchronize()
+        self.ctx.check(lib.qlg_plan_super_block(self.ctx.h, len(terms), nl, left, nr, right, C.byref(sd),
+                                                PRECISIONS[precision], C.byref(h)))
+
+    def _plan_mpo_form(self, sd, precision, h):
+        lib = self.ctx.lib
+        descs = [t.descriptor() for t in self.ops]
         torch.cuda.synchronize()
         if len(self.ops) == 4:
             rc = lib.qlg_plan_two_site(self.ctx.h, *[C.byref(d) for d in descs], C.byref(sd),
@@ -75,6 +105,9 @@ class EffectiveHamiltonian:
         else:
             raise ValueError("eff_ham must be [L, W_l, W_r, R] or [L, W, R]")
         self.ctx.check(rc)
+
+    def _finish(self, h, state):
+        lib = self.ctx.lib
         self.h = h
         self.lib = lib
         nb, size = C.c_int64(), C.c_int64()
@@ -199,7 +232,13 @@ def eff_ham_mul_single_site_state(eff_ham: Sequence[BlockTensor], state: BlockTe
 
 def LanczosSolver(eff_ham, init_state: BlockTensor, matvec=None, params: LanczosParams = LanczosParams(),
                   precision: str = "fp64") -> LanczosRes:
-    """``lanczos_vmps_solver_impl.h:114-248``; ``eff_ham`` may be an EffectiveHamiltonian."""
+    """``lanczos_vmps_solver_impl.h:114-248``; ``eff_ham`` may be an EffectiveHamiltonian.
+
+    The DMRG overload ``LanczosSolver(SuperBlockHamiltonianTerms&, TenT*, const
+    LanczosParams&)`` (``lanczos_dmrg_solver_impl.h:61-196``) is the call with
+    the term list as ``eff_ham`` and the parameters in the third position."""
+    if isinstance(matvec, LanczosParams):   # DMRG form: (terms, state, params)
+        params, matvec = matvec, None
     H = eff_ham if isinstance(eff_ham, EffectiveHamiltonian) else EffectiveHamiltonian(eff_ham, init_state,
                                                                                        precision)
     return H.lanczos(init_state, params)
