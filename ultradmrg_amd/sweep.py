@@ -194,6 +194,12 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         pad = torch.arange(b, device=dev)[None, :] >= ns[:, None]
         G.diagonal(dim1=1, dim2=2)[pad] = -1.0     # padded rows separate below the spectrum
         w, V = torch.linalg.eigh(G)
+        if G.is_complex() and bool(torch.isnan(V).any()):
+            # the complex128 batched eigensolver returns NaN eigenvectors when a
+            # Householder column of the padded Gram matrix is exactly zero (a
+            # sector with a single row): such buckets are solved on the host
+            wc, Vc = torch.linalg.eigh(G.cpu())
+            w, V = wc.to(dev), Vc.to(dev)
         lam = w.flip(-1).clamp_min(0.0)
         E = V.flip(-1)
         S = lam.sqrt()
