@@ -386,6 +386,7 @@ class SweepParams:
     lancz: LanczosParams
     precision: str = "fp64"
     verbose: bool = True
+    checkpoint_dir: Optional[str] = None   # MPS dumped here after every sweep (and on a stop signal)
 
 
 @dataclass
@@ -415,6 +416,8 @@ class TwoSiteVMPS:
             self.renvs[i] = env_update(self.renvs[i - 1], mps[n - i], mpo[n - i], "r")
         self.lenvs[0] = trivial_env(mps[0].axes[0], mpo[0].axes[0], dev, dt)
         self.stats = SweepStats()
+        self.sweeps_done = 0
+        self._flags = None
 
     def update(self, direction: str, target: int) -> float:
         n = len(self.mps)
@@ -464,23 +467,71 @@ class TwoSiteVMPS:
                   file=sys.stderr, flush=True)
         return res.gs_eng
 
-    def sweep(self) -> float:
+    def sweep(self) -> Optional[float]:
+        """One right and one left half-sweep; returns the last update's energy, or
+        None after an emergency stop (the MPS is dumped mid-sweep)."""
         n = len(self.mps)
         e0 = None
         t0 = time.perf_counter()
-        for i in range(0, n - 2):
-            e0 = self.update("r", i)
-        for i in range(n - 1, 1, -1):
-            e0 = self.update("l", i)
+        for direction, sites in (("r", range(0, n - 2)), ("l", range(n - 1, 1, -1))):
+            for i in sites:
+                e0 = self.update(direction, i)
+                if self._flags is not None and self._flags.emergency:
+                    from .checkpoint import save_mps
+                    save_mps(self.p.checkpoint_dir, self.mps, {"complete_sweep": False,
+                                                               "sweeps_done": self.sweeps_done,
+                                                               "stopped_at": [direction, i], "energy": e0})
+                    return None
         torch.cuda.synchronize()
         self.stats.sweep_seconds.append(time.perf_counter() - t0)
         return e0
 
     def run(self) -> float:
+        """``params.sweeps`` sweeps.  With ``checkpoint_dir`` set, the MPS is dumped
+        after every sweep; SIGUSR1 / SIGUSR2 stop after the current sweep and
+        SIGINT / SIGTERM after the current update (``dmrg_impl.h:478-502``), in
+        both cases with the state dumped first."""
+        from .checkpoint import StopFlags, save_mps
         e0 = None
-        for _ in range(self.p.sweeps):
-            e0 = self.sweep()
+        flags = StopFlags().install() if self.p.checkpoint_dir else None
+        self._flags = flags
+        try:
+            for k in range(self.p.sweeps):
+                e0 = self.sweep()
+                if e0 is None:        # emergency stop inside the sweep (already dumped)
+                    break
+                self.sweeps_done += 1
+                if self.p.checkpoint_dir:
+                    save_mps(self.p.checkpoint_dir, self.mps,
+                             {"complete_sweep": True, "sweeps_done": self.sweeps_done, "energy": e0})
+                if flags and flags.stop:
+                    break
+        finally:
+            if flags:
+                flags.uninstall()
+            self._flags = None
         return e0
+
+    @classmethod
+    def resume(cls, checkpoint_dir: str, mpo: List[BlockTensor], params: SweepParams, device="cuda"):
+        """Continue from a dump.  A dump taken inside a sweep is right-canonicalised
+        (centre moved to site 0) first; the state itself is unchanged."""
+        from .checkpoint import load_mps
+        mps, man = load_mps(checkpoint_dir, device)
+        if not man.get("complete_sweep", False):
+            center_to_zero(mps)
+        drv = cls(mps, mpo, params)
+        drv.sweeps_done = int(man.get("sweeps_done", 0))
+        return drv
+
+
+def center_to_zero(mps: List[BlockTensor], start: Optional[int] = None):
+    """Right-canonicalise sites start .. 1 (default N-1 .. 1) by SVD without truncation."""
+    n = len(mps)
+    for i in range(n - 1 if start is None else start, 0, -1):
+        r = block_svd(mps[i], 1, divergence(mps[i]), 0.0, 1, 10 ** 9)
+        mps[i] = r.vt
+        mps[i - 1] = contract(mps[i - 1], contract(r.u, r.s, 1), 1)
 
 
 def TwoSiteFiniteVMPS(mps, mpo, params: SweepParams):

@@ -30,8 +30,8 @@ import numpy as np
 import torch
 
 from .solver import EffectiveHamiltonian, LanczosParams
-from .sweep import block_svd, divergence, trivial_env
-from .tensor import IN, OUT, Axis, BlockTensor, contract, env_update
+from .sweep import block_svd, center_to_zero, divergence, trivial_env
+from .tensor import OUT, Axis, BlockTensor, contract, env_update
 
 
 @dataclass
@@ -140,15 +140,6 @@ class TwoSiteTDVP:
         center_to_zero(self.mps)
         self.rebuild_envs()
         return local_expectations(self.mps, op)
-
-
-def center_to_zero(mps: List[BlockTensor], start: Optional[int] = None):
-    """Right-canonicalise sites start .. 1 (default N-1 .. 1) by SVD without truncation."""
-    n = len(mps)
-    for i in range(n - 1 if start is None else start, 0, -1):
-        r = block_svd(mps[i], 1, divergence(mps[i]), 0.0, 1, 10 ** 9)
-        mps[i] = r.vt
-        mps[i - 1] = contract(mps[i - 1], contract(r.u, r.s, 1), 1)
 
 
 def apply_local_op(mps: List[BlockTensor], op: np.ndarray, site: int):
