@@ -223,6 +223,24 @@ int qlg_lanczos_gs(qlg_plan plan, void *psi_inout,
 int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau,
                       const qlg_lanczos_params *params, int64_t *iters);
 
+/* ---- batched Hermitian eigen-decomposition (block SVD) -------------------------
+ * The qlten SVD of the two-site state (two_site_update_finite_vmps_impl.h:302-307,
+ * dmrg_impl.h:249-254, two_site_update_finite_tdvp_impl.h:339-344) is computed
+ * from the eigen-decompositions of its sector Gram matrices; this entry point
+ * solves them.  Group k holds batch matrices of order n, row-major and
+ * Hermitian, at a (dtype QLG_F64 or QLG_C128); on return row j of each matrix
+ * holds eigenvector j (complex-conjugated for QLG_C128) and w[0..n) its
+ * eigenvalues in ascending order.  Divide and conquer; groups run
+ * concurrently.  The call returns after the work is complete (it checks the
+ * solver's convergence flags); QLG_ERR_NUMERIC if a matrix did not converge. */
+typedef struct {
+  int64_t n;      /* order of each matrix   */
+  int64_t batch;  /* matrices in the group  */
+  void *a;        /* [batch][n][n] device   */
+  double *w;      /* [batch][n] device      */
+} qlg_eigh_group;
+int qlg_eigh_batched(qlg_ctx ctx, int dtype, int64_t ngroups, const qlg_eigh_group *groups);
+
 /* ---- host helpers (FP64) ------------------------------------------------------
  * qlg_tridiag_gs: TridiagGsSolver, lanczos_vmps_solver_impl.h:306-355
  * (lowest eigenpair of the symmetric tridiagonal T(a[0..n), b[0..n-1));

@@ -218,3 +218,33 @@ def test_checkpoint_resume(tmp_path):
     back = TwoSiteVMPS.resume(d2, mpo, prm(3))
     e = back.run()
     assert abs(e - energies[-1]) < 1e-9
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_eigh_batched_c_abi(cplx):
+    """qlg_eigh_batched (the block SVD's Gram eigensolver) on several groups at
+    once: eigenvalues against torch.linalg.eigh, residual |A v - lambda v| and
+    orthonormality of the returned vectors, including a diagonal (zero
+    off-diagonal) matrix and order-2 matrices."""
+    from ultradmrg_amd.sweep import eigh_batched
+    print(f"[case] eigh batched complex={cplx}", flush=True)
+    dt = torch.complex128 if cplx else torch.float64
+    g = torch.Generator(device="cpu").manual_seed(17)
+    mats = []
+    for B, n in ((3, 2), (5, 40), (2, 257), (1, 600)):
+        X = torch.randn(B, n, n + 3, generator=g, dtype=torch.float64)
+        if cplx:
+            X = X + 1j * torch.randn(B, n, n + 3, generator=g, dtype=torch.float64)
+        X = X * torch.logspace(0, -5, n, dtype=torch.float64)[None, :, None]
+        mats.append((X @ X.mT.conj()).to(dt))
+    mats[1][0] = torch.diag(torch.linspace(0.1, 2.0, 40, dtype=torch.float64)).to(dt)
+    dev = [m.cuda() for m in mats]
+    res = eigh_batched([m.clone() for m in dev])
+    for A, (w, V) in zip(dev, res):
+        w0 = torch.linalg.eigh(A.cpu())[0]
+        scale = float(w0.abs().max())
+        assert float((w.cpu() - w0).abs().max()) < 1e-13 * scale
+        r = A @ V - V * w[:, None, :].to(dt)
+        assert float(r.abs().max()) < 1e-12 * scale
+        eye = torch.eye(A.shape[1], dtype=dt, device="cuda")
+        assert float((V.mH @ V - eye).abs().max()) < 1e-12

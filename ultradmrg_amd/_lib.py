@@ -30,7 +30,8 @@ EXPORTS = [
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
     "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
     "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
-    "qlg_linear_combine", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_tridiag_gs", "qlg_tridiag_expme1",
+    "qlg_linear_combine", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_eigh_batched", "qlg_tridiag_gs",
+    "qlg_tridiag_expme1",
 ]
 
 
@@ -52,6 +53,10 @@ class Bsten(C.Structure):
 
 class Term(C.Structure):
     _fields_ = [("op0", C.POINTER(Bsten)), ("op1", C.POINTER(Bsten))]
+
+
+class EighGroup(C.Structure):
+    _fields_ = [("n", C.c_int64), ("batch", C.c_int64), ("a", C.c_void_p), ("w", C.c_void_p)]
 
 
 class LanczosParams(C.Structure):
@@ -110,6 +115,7 @@ def lib():
         "qlg_linear_combine": (C.c_int, [vp, C.c_int, i64, i64, dp, C.POINTER(vp), dp, vp]),
         "qlg_lanczos_gs": (C.c_int, [vp, vp, C.POINTER(LanczosParams), C.POINTER(LanczosResult)]),
         "qlg_lanczos_expmv": (C.c_int, [vp, vp, C.c_double, C.POINTER(LanczosParams), C.POINTER(i64)]),
+        "qlg_eigh_batched": (C.c_int, [vp, C.c_int, i64, C.POINTER(EighGroup)]),
         "qlg_tridiag_gs": (C.c_int, [dp, dp, i64, C.c_char, dp, dp]),
         "qlg_tridiag_expme1": (C.c_int, [dp, dp, i64, C.c_double, dp]),
     }

@@ -20,6 +20,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libqlgpu.so")
 OBJDIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
+ROCM_LIB = "/opt/rocm/lib"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           f"-I{os.path.join(ROOT, 'include')}"]
@@ -64,7 +65,8 @@ def build(jobs: int = 8, debug: bool = False, verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(lambda s: compile_one(s, debug), srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,
+           f"-L{ROCM_LIB}", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

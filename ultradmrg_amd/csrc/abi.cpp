@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 
+#include "eigh.h"
 #include "krylov.h"
 #include "plan.h"
 #include "tridiag.h"
@@ -75,6 +76,8 @@ int qlg_ctx_destroy(qlg_ctx ctx) {
   if (!ctx) return QLG_OK;
   (void)hipSetDevice(ctx->c.device);
   if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  destroy_eigh_pool(ctx->c.eig);
+  ctx->c.eig = nullptr;
   if (ctx->c.host_scalars) (void)hipHostFree(ctx->c.host_scalars);
   if (ctx->c.own_stream && ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
@@ -368,6 +371,11 @@ int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau, const qlg_lanc
       throw Error(QLG_ERR_UNSUPPORTED, "qlg_lanczos_expmv runs in FP64 (complex data)");
     lanczos_expmv(p, static_cast<double *>(psi_inout), tau, *params, *iters);
   });
+}
+
+int qlg_eigh_batched(qlg_ctx ctx, int dtype, int64_t ngroups, const qlg_eigh_group *groups) {
+  if (!ctx) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] { eigh_batched(ctx->c, dtype, ngroups, groups); });
 }
 
 int qlg_tridiag_gs(const double *a, const double *b, int64_t n, char jobz, double *eng, double *vec) {

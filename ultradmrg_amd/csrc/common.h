@@ -60,6 +60,9 @@ struct DevBuf {
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+struct EighPool;
+void destroy_eigh_pool(EighPool *p);
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -68,6 +71,7 @@ struct Ctx {
   DevBuf partials;   // reduction scratch
   DevBuf scalars;    // device-side scalar results
   double *host_scalars = nullptr;  // pinned, 64 doubles
+  EighPool *eig = nullptr;         // lanes of the batched eigensolver, created on first use
 };
 
 // ---------------------------------------------------------------------------

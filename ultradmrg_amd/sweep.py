@@ -29,7 +29,8 @@ import torch
 
 from .mpo import Model, physical_axis
 from .solver import EffectiveHamiltonian, LanczosParams
-from .tensor import IN, OUT, Axis, BlockTensor, _flow, contract, env_update
+from . import _lib as L
+from .tensor import IN, OUT, Axis, BlockTensor, _flow, contract, default_context, env_update
 
 
 def divergence(t: BlockTensor) -> Tuple[int, ...]:
@@ -63,11 +64,14 @@ class SVDResult:
     entropy: float
 
 
-# padded sizes of the batched eigen-decompositions.  rocSOLVER's tridiagonal
-# reduction is latency bound (hundreds of small launches per call), so few
-# buckets with coarse steps beat tight padding: at D=2000 (sectors up to ~750)
-# (256, 768) costs 37 ms per SVD against 64 ms for 1.5x steps from 64.
-_BUCKETS = (256, 768, 2048, 4096, 8192)
+# padded sizes of the batched eigen-decompositions.  The divide-and-conquer
+# solver's tridiagonal reduction is bound by launch latency: a batch costs
+# ~22 us x its padded order x (1 + 0.045 x batch) on MI355X, and the batches run
+# concurrently on four lanes.  So small sectors share one coarse bucket, while
+# above 512 the steps are fine enough that the largest sector, which bounds
+# the call, is padded by at most 64.  Sectors with one row need no solver.
+_BUCKETS = (1, 256, 512, 576, 640, 704, 768, 832, 896, 960, 1024, 1152, 1280, 1408, 1536, 1792, 2048,
+            2560, 3072, 4096, 6144, 8192)
 _SV_NOISE = 1e-6   # singular values at or below this fraction of their sector's largest count as zero
 
 
@@ -157,6 +161,78 @@ def _svd_structure(t: BlockTensor, ldims: int, ldiv) -> _SvdStructure:
     return st
 
 
+_UNITARY: Dict[tuple, torch.Tensor] = {}
+
+
+def _random_unitary(b: int, dtype, device, attempt: int = 0) -> torch.Tensor:
+    key = (b, dtype, str(device), attempt)
+    q = _UNITARY.get(key)
+    if q is None:
+        gen = torch.Generator(device="cpu").manual_seed(4321 + 7919 * attempt + b)
+        z = torch.complex(torch.randn(b, b, generator=gen, dtype=torch.float64),
+                          torch.randn(b, b, generator=gen, dtype=torch.float64))
+        q = _UNITARY[key] = torch.linalg.qr(z.to(device))[0].to(dtype)
+    return q
+
+
+def eigh_batched(mats: List[torch.Tensor]) -> List[Tuple[torch.Tensor, torch.Tensor]]:
+    """Eigen-decompositions of batches of Hermitian matrices ([B, n, n] each, all
+    of one dtype, n > 1) through ``qlg_eigh_batched``: one library call for all
+    batches.  Returns (w [B, n] ascending, V [B, n, n] with eigenvectors as
+    columns).  The inputs are consumed (they hold the eigenvectors on return)."""
+    if not mats:
+        return []
+    dev, dt = mats[0].device, mats[0].dtype
+    ctx = default_context(dev.index or 0)
+    groups = (L.EighGroup * len(mats))()
+    ws = []
+    for g, A in zip(groups, mats):
+        assert A.is_contiguous() and A.dim() == 3 and A.shape[1] == A.shape[2] and A.dtype == dt
+        w = torch.empty(A.shape[0], A.shape[1], dtype=torch.float64, device=dev)
+        g.n, g.batch, g.a, g.w = A.shape[1], A.shape[0], A.data_ptr(), w.data_ptr()
+        ws.append(w)
+    torch.cuda.synchronize()     # the library works on its own streams; it returns after the work is done
+    ctx.check(ctx.lib.qlg_eigh_batched(ctx.h, L.QLG_C128 if dt.is_complex else L.QLG_F64, len(mats), groups))
+    # row j of each buffer is eigenvector j (conjugated for complex data)
+    return [(w, A.mT.conj()) for w, A in zip(ws, mats)]
+
+
+def _eigh_grams(grams: Dict[int, torch.Tensor]) -> Dict[int, Tuple[torch.Tensor, torch.Tensor]]:
+    """Eigen-decompositions of the padded Gram batches of a block SVD.
+
+    Complex batches are rotated by a fixed random unitary, Q G Q^H, before the
+    solve and the vectors rotated back: the complex divide-and-conquer solver
+    returns NaN vectors when the tridiagonal reduction meets exactly-zero
+    sub-columns, which block-diagonal (padded or reducible) Gram matrices
+    produce.  A batch that still comes back with NaN is solved again with
+    another rotation; a third failure raises."""
+    out, todo = {}, []
+    for b, G in grams.items():
+        if b == 1:
+            out[b] = (G[:, :, 0].real.contiguous(), torch.ones_like(G))
+        else:
+            todo.append(b)
+    if not todo:
+        return out
+    if not grams[todo[0]].is_complex():
+        for b, r in zip(todo, eigh_batched([grams[b].clone() for b in todo])):
+            out[b] = r
+        return out
+    for attempt in range(3):
+        qs = {b: _random_unitary(b, grams[b].dtype, grams[b].device, attempt) for b in todo}
+        res = eigh_batched([(qs[b] @ grams[b] @ qs[b].mH).contiguous() for b in todo])
+        left = []
+        for b, (w, V) in zip(todo, res):
+            if bool(torch.isnan(V).any() or torch.isnan(w).any()):
+                left.append(b)
+            else:
+                out[b] = (w, qs[b].mH @ V)
+        todo = left
+        if not todo:
+            return out
+    raise RuntimeError(f"block SVD: the eigensolver returned NaN for Gram batches of order {todo}")
+
+
 def _complete_rows(A: torch.Tensor, bad: torch.Tensor) -> torch.Tensor:
     """Replace the rows of A flagged in ``bad`` by unit vectors orthogonal to the others."""
     k, n = A.shape
@@ -186,20 +262,22 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     flat[st.dst] = t.data
     per = {}   # bucket -> (E [B, b, b] descending eigenvectors, S [B, b], P = E^T X / S [B, b, mpad])
     svals = []
+    Xs, grams = {}, {}
     for b, sis in st.buckets.items():
         mp, B = st.mpad[b], len(sis)
         X = flat[st.bucket_base[b]:st.bucket_base[b] + B * b * mp].view(B, b, mp)
         G = X @ X.mT.conj()
         ns = torch.tensor([st.sect[si]["n"] for si in sis], device=dev)
         pad = torch.arange(b, device=dev)[None, :] >= ns[:, None]
-        G.diagonal(dim1=1, dim2=2)[pad] = -1.0     # padded rows separate below the spectrum
-        w, V = torch.linalg.eigh(G)
-        if G.is_complex() and bool(torch.isnan(V).any()):
-            # the complex128 batched eigensolver returns NaN eigenvectors when a
-            # Householder column of the padded Gram matrix is exactly zero (a
-            # sector with a single row): such buckets are solved on the host
-            wc, Vc = torch.linalg.eigh(G.cpu())
-            w, V = wc.to(dev), Vc.to(dev)
+        # padded rows separate below the spectrum (distinct values for complex
+        # data, whose solver is given a rotated, dense matrix)
+        padv = -1.0 - torch.arange(b, device=dev, dtype=torch.float64) / b if G.is_complex() else -1.0
+        G.diagonal(dim1=1, dim2=2)[pad] = (padv.expand(len(sis), b)[pad].to(G.dtype) if G.is_complex() else padv)
+        Xs[b], grams[b] = X, G
+    eig = _eigh_grams(grams)
+    for b in st.buckets:
+        X = Xs[b]
+        w, V = eig[b]
         lam = w.flip(-1).clamp_min(0.0)
         E = V.flip(-1)
         S = lam.sqrt()
