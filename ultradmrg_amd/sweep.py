@@ -65,13 +65,12 @@ class SVDResult:
 
 
 # padded sizes of the batched eigen-decompositions.  The divide-and-conquer
-# solver's tridiagonal reduction is bound by launch latency: a batch costs
-# ~22 us x its padded order x (1 + 0.045 x batch) on MI355X, and the batches run
-# concurrently on four lanes.  So small sectors share one coarse bucket, while
-# above 512 the steps are fine enough that the largest sector, which bounds
-# the call, is padded by at most 64.  Sectors with one row need no solver.
-_BUCKETS = (1, 256, 512, 576, 640, 704, 768, 832, 896, 960, 1024, 1152, 1280, 1408, 1536, 1792, 2048,
-            2560, 3072, 4096, 6144, 8192)
+# solver's tridiagonal reduction is bound by host launch latency: a batch
+# costs ~22 us x its padded order x (1 + 0.045 x batch) on MI355X.  Batches run
+# concurrently on four lanes, but lanes contend for the launch path, so a few
+# coarse buckets beat fine ones (30 ms against 38 ms per D=2000 SVD for 64-wide
+# steps above 512).  Sectors with one row need no solver.
+_BUCKETS = (1, 256, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192)
 _SV_NOISE = 1e-6   # singular values at or below this fraction of their sector's largest count as zero
 
 
@@ -87,20 +86,31 @@ class _SvdStructure:
     flat gather index that places every element of the tensor into padded,
     per-size-bucket batches of sector matrices.  Each sector matrix is stored
     with its shorter side first (transposed when it has more rows than
-    columns), so its Gram matrix on that side is one batched product."""
+    columns), so its Gram matrix on that side is one batched product.  The
+    host builds a per-block table; the per-element index is computed on the
+    device from it."""
 
     def __init__(self, t: BlockTensor, ldims: int, ldiv):
         left, right = t.axes[:ldims], t.axes[ldims:]
+        sgn = [-1 if ax.dir == OUT else 1 for ax in left]
         groups: Dict[tuple, dict] = {}
         for i, c in enumerate(t.coords):
-            q = tuple(ldiv)
-            for ax, s in zip(left, c[:ldims]):
-                sgn = -1 if ax.dir == OUT else 1
-                q = tuple(a + sgn * b for a, b in zip(q, ax.qns[s]))
-            g = groups.setdefault(q, {"rows": {}, "cols": {}, "blocks": []})
+            q = list(ldiv)
+            for sg, ax, sc in zip(sgn, left, c[:ldims]):
+                for j, v in enumerate(ax.qns[sc]):
+                    q[j] += sg * v
+            g = groups.setdefault(tuple(q), {"rows": {}, "cols": {}, "blocks": []})
             lk, rk = c[:ldims], c[ldims:]
-            g["rows"].setdefault(lk, int(np.prod([ax.degs[s] for ax, s in zip(left, lk)])))
-            g["cols"].setdefault(rk, int(np.prod([ax.degs[s] for ax, s in zip(right, rk)])))
+            if lk not in g["rows"]:
+                nr = 1
+                for ax, sc in zip(left, lk):
+                    nr *= ax.degs[sc]
+                g["rows"][lk] = nr
+            if rk not in g["cols"]:
+                nc = 1
+                for ax, sc in zip(right, rk):
+                    nc *= ax.degs[sc]
+                g["cols"][rk] = nc
             g["blocks"].append((i, lk, rk))
         self.qs = sorted(groups)
         self.sect = []
@@ -125,8 +135,10 @@ class _SvdStructure:
             self.buckets.setdefault(_bucket(sc["n"]), []).append(si)
         self.mpad = {b: max(16, -(-max(self.sect[si]["m"] for si in sis) // 16) * 16)
                      for b, sis in self.buckets.items()}
-        # gather: element e of t.data -> flat position in the concatenated bucket batches
-        dst = np.full(t.data.numel(), -1, dtype=np.int64)   # -1: element of no block
+        # per-block table: data offset, rows, cols, batch base, row/col origin in
+        # the sector matrix, transposed flag, padded long side
+        nb = len(t.coords)
+        tab = np.zeros((nb, 8), dtype=np.int64)
         base = 0
         self.bucket_base = {}
         for b, sis in self.buckets.items():
@@ -136,16 +148,24 @@ class _SvdStructure:
                 sc = self.sect[si]
                 sbase = base + j * b * mp
                 for (i, lk, rk) in sc["blocks"]:
-                    nr, nc = sc["rows"][lk], sc["cols"][rk]
-                    rr = sc["ro"][lk] + np.arange(nr)[:, None]
-                    cc = sc["co"][rk] + np.arange(nc)[None, :]
-                    pos = sbase + (cc * mp + rr if sc["tr"] else rr * mp + cc)
-                    off = t.offsets[i]
-                    dst[off:off + nr * nc] = pos.reshape(-1)
+                    tab[i] = (t.offsets[i], sc["rows"][lk], sc["cols"][rk], sbase, sc["ro"][lk], sc["co"][rk],
+                              int(sc["tr"]), mp)
             base += len(sis) * b * mp
         self.total = base
-        dst[dst < 0] = base          # one trash slot past the batches
-        self.dst = torch.from_numpy(dst).to(t.data.device)
+        dev = t.data.device
+        sizes = tab[:, 1] * tab[:, 2]
+        ntot = int(sizes.sum())
+        T = torch.from_numpy(np.ascontiguousarray(tab.T)).to(dev)           # [8, nblocks]
+        size = T[1] * T[2]
+        blk = torch.repeat_interleave(torch.arange(nb, device=dev), size, output_size=ntot)
+        Tb = T[:, blk]                                                        # [8, elements]
+        loc = torch.arange(ntot, device=dev) - (torch.cumsum(size, 0) - size)[blk]
+        r = torch.div(loc, Tb[2], rounding_mode="floor")
+        rr, cc = Tb[4] + r, Tb[5] + (loc - r * Tb[2])
+        pos = Tb[3] + torch.where(Tb[6] > 0, cc * Tb[7] + rr, rr * Tb[7] + cc)
+        dst = torch.full((t.data.numel(),), base, dtype=torch.int64, device=dev)   # base: one trash slot
+        dst[Tb[0] + loc] = pos
+        self.dst = dst
 
 
 _SVD_CACHE: Dict[tuple, _SvdStructure] = {}
@@ -291,10 +311,11 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         svals.append(S)
     # one host transfer for the truncation decision
     sv_all = [s.double().cpu().numpy() for s in svals]
-    sv, owner = [], []
+    sv, owner, sv_host = [], [], {}
     for (b, sis), arr in zip(st.buckets.items(), sv_all):
         for j, si in enumerate(sis):
             n = st.sect[si]["n"]
+            sv_host[si] = arr[j, :n]
             sv.append(arr[j, :n])
             owner.append(np.full(n, si))
     sv = np.concatenate(sv)
@@ -332,7 +353,7 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         else:
             U = E[j, :n, :k]
             Vh = P[j, :k, :m]
-        if float(Sk[-1]) == 0.0:
+        if sv_host[si][k - 1] == 0.0:
             bad = Sk == 0
             if sc["tr"]:
                 U = _complete_rows(U.mT.contiguous(), bad).mT
