@@ -223,6 +223,28 @@ int qlg_lanczos_gs(qlg_plan plan, void *psi_inout,
 int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau,
                       const qlg_lanczos_params *params, int64_t *iters);
 
+/* ---- independent chains in one launch ------------------------------------------
+ * The batching form of the path: several effective Hamiltonians (each a
+ * two-site, one-site or super-block plan of its own chain and site) are solved
+ * together.  qlg_plan_batch copies the member plans' operators into one plan
+ * whose every matvec stage is a single grouped launch over all members (the
+ * members may be destroyed afterwards).  Its vectors are the members' vectors
+ * concatenated: member m occupies [offsets[m], offsets[m+1]) of a batch
+ * vector, in the member's own layout (qlg_plan_batch_offsets).  qlg_matvec
+ * applies all members at once.  qlg_lanczos_gs_batch runs LanczosSolver
+ * (lanczos_vmps_solver_impl.h:114-248) for every member in lock step, each
+ * with its own scalars, stop rule and result (res[m]); the iteration counts
+ * and energies are those of separate solves.  qlg_plan_use_graphs(plan, 1)
+ * captures each Lanczos iteration's device work as a hipGraph on first use
+ * and replays it in later solves on the same plan.  Real (QLG_F64) FP64
+ * members only.  No counterpart in the reference (which solves one site of
+ * one chain at a time).                                                     */
+int qlg_plan_batch(qlg_ctx ctx, int64_t nplans, const qlg_plan *plans, qlg_plan *out);
+int qlg_plan_batch_offsets(qlg_plan plan, int64_t *nplans, int64_t *offsets /* nplans + 1, may be NULL */);
+int qlg_plan_use_graphs(qlg_plan plan, int on);
+int qlg_lanczos_gs_batch(qlg_plan plan, void *psi_inout, const qlg_lanczos_params *params,
+                         qlg_lanczos_result *res);
+
 /* ---- batched Hermitian eigen-decomposition (block SVD) -------------------------
  * The qlten SVD of the two-site state (two_site_update_finite_vmps_impl.h:302-307,
  * dmrg_impl.h:249-254, two_site_update_finite_tdvp_impl.h:339-344) is computed

@@ -173,6 +173,49 @@ int qlg_plan_super_block(qlg_ctx ctx, int64_t ngroups, const int64_t *nleft, con
   });
 }
 
+int qlg_plan_batch(qlg_ctx ctx, int64_t nplans, const qlg_plan *plans, qlg_plan *out) {
+  if (!ctx || !out || !plans || nplans < 1) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    std::vector<Plan *> members(nplans);
+    for (int64_t i = 0; i < nplans; ++i) {
+      QLG_CHECK(plans[i] != nullptr, "qlg_plan_batch: null member plan");
+      members[i] = plans[i]->p.get();
+    }
+    auto h = std::make_unique<qlg_plan_s>();
+    h->p.reset(build_batch_plan(&ctx->c, nplans, members.data()));
+    *out = h.release();
+  });
+}
+
+int qlg_plan_batch_offsets(qlg_plan plan, int64_t *nplans, int64_t *offsets) {
+  if (!plan || !nplans) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.batch_off.size() >= 2, "not a batch plan");
+    *nplans = (int64_t)p.batch_off.size() - 1;
+    if (offsets)
+      for (size_t i = 0; i < p.batch_off.size(); ++i) offsets[i] = p.batch_off[i];
+  });
+}
+
+int qlg_plan_use_graphs(qlg_plan plan, int on) {
+  if (!plan) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.batch_off.size() >= 2, "graph capture applies to batch plans");
+    p.use_graphs = on != 0;
+    if (!p.use_graphs) p.drop_graphs();
+  });
+}
+
+int qlg_lanczos_gs_batch(qlg_plan plan, void *psi_inout, const qlg_lanczos_params *params,
+                         qlg_lanczos_result *res) {
+  if (!plan || !psi_inout || !params || !res) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] { lanczos_gs_batch(p, static_cast<double *>(psi_inout), *params, res); });
+}
+
 int qlg_plan_destroy(qlg_plan plan) {
   if (!plan) return QLG_OK;
   (void)hipSetDevice(plan->p->ctx->device);

@@ -77,7 +77,7 @@ std::vector<double> mpo_dense(const Ten &t, hipStream_t st, int ed, std::vector<
 void finalize(Plan &p, int nst, const int64_t *slot_elems, bool cplx) {
   for (int i = 0; i < nst; ++i) validate_stage(p.st[i], slot_elems);
   for (int i = 0; i < nst; ++i) {
-    autotune_tiles(p.st[i], cplx);
+    model_tiles(p.st[i], cplx);
     p.st[i].upload(p.ctx->stream);
   }
   p.nstages = nst;

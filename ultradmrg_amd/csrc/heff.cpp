@@ -18,6 +18,9 @@
 #include <cstdlib>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
 #include <set>
 
 #include "plan.h"
@@ -571,6 +574,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   for (int i = 0; i < 3; ++i) {
     if (pinned[i][0] > 0) {
       p.st[i].make_tiles(pinned[i][0], pinned[i][1]);
+      p.tiles_pinned = true;
     } else if (i == 1) {
       // MPO stage: HBM-bound.  X is re-read once per row tile, so the row
       // tile covers K_out where the MFMA padding stays cheap; the cost counts
@@ -590,7 +594,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       }
       p.st[i].make_tiles(bm_best, bm_best == 64 ? 64 : 128);
     } else {
-      autotune_tiles(p.st[i], p.dtype == QLG_C128);
+      model_tiles(p.st[i], p.dtype == QLG_C128);   // replaced by measured shapes on the first matvec
     }
     p.st[i].upload(ctx->stream);
   }
@@ -642,6 +646,32 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       std::fprintf(stderr, "[qlg plan] stage %d (%s): tasks %zu tiles %zu tile %dx%d gflop %.3f\n", i + 1, names[i],
                    i == 1 ? s.chan.size() : s.tasks.size(), s.tiles.size(), s.bm, s.bn, s.flops / 1e9);
     }
+    for (int i : {0, 2}) {
+      // tile efficiency (useful / padded MFMA work) and the task shape spread
+      const Stage &s = p.st[i];
+      double useful = 0, padded = 0;
+      std::vector<int> ms, ns, ks;
+      for (const GemmTask &k : s.tasks) {
+        int64_t K = 0, Kp = 0;
+        for (int q = 0; q < k.seg_count; ++q) {
+          K += s.segs[k.seg_begin + q].K;
+          Kp += (s.segs[k.seg_begin + q].K + 15) / 16 * 16;
+        }
+        useful += 2.0 * k.M * k.N * K;
+        padded += 2.0 * double((k.M + s.bm - 1) / s.bm * s.bm) * double((k.N + s.bn - 1) / s.bn * s.bn) * Kp;
+        ms.push_back(k.M);
+        ns.push_back(k.N);
+        ks.push_back((int)K);
+      }
+      auto pct = [](std::vector<int> v, double q) {
+        if (v.empty()) return 0;
+        std::sort(v.begin(), v.end());
+        return v[std::min(v.size() - 1, size_t(q * v.size()))];
+      };
+      std::fprintf(stderr, "[qlg plan] stage %d tile efficiency %.3f; M p10/50/90 %d/%d/%d, N %d/%d/%d, K %d/%d/%d\n",
+                   i + 1, padded > 0 ? useful / padded : 0.0, pct(ms, .1), pct(ms, .5), pct(ms, .9), pct(ns, .1),
+                   pct(ns, .5), pct(ns, .9), pct(ks, .1), pct(ks, .5), pct(ks, .9));
+    }
     std::fprintf(stderr, "[qlg plan] vector %lld, T1 %lld, Y %lld, C %lld, L stacks %lld, R panels %lld, algo gflop %.3f\n",
                  (long long)p.vec.size, (long long)p.x_elems, (long long)p.y_elems, (long long)p.c_elems,
                  (long long)lint_elems, (long long)rint_elems, p.algo_flops / 1e9);
@@ -670,6 +700,14 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
 Plan::~Plan() {
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  drop_graphs();
+  if (host_seg) (void)hipHostFree(host_seg);
+}
+
+void Plan::drop_graphs() {
+  for (auto &g : step_graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  step_graphs.clear();
 }
 
 void Plan::to_fused(const double *canon, double *fused) {
@@ -694,9 +732,8 @@ void Plan::prepare_reduced() {
   reduced_ready = true;
 }
 
-void Plan::matvec_fused(const double *in, double *out) {
+Slots Plan::slots(const double *in, double *out) {
   const bool reduced = precision != QLG_PREC_F64;
-  if (reduced) prepare_reduced();
   Slots s{};
   s.p[SLOT_IN] = const_cast<double *>(in);
   s.p[SLOT_OUT] = out;
@@ -706,35 +743,149 @@ void Plan::matvec_fused(const double *in, double *out) {
   s.p[SLOT_Y] = reduced ? reinterpret_cast<double *>(Y32.p) : Y.as<double>();
   s.p[SLOT_C] = reduced ? reinterpret_cast<double *>(C32.p) : C.as<double>();
   s.crows = st[2].d_crows.as<int64_t>();
+  return s;
+}
+
+void Plan::launch_stage(int i, const Slots &s) {
+  const bool reduced = precision != QLG_PREC_F64, cplx = dtype == QLG_C128;
   hipStream_t stream = ctx->stream;
+  Stage &g = st[i];
+  if (i == 1) {
+    launch_mpo_stage(g.bm, g.bn, cplx, reduced, g.d_tiles.as<GemmTile>(), g.d_chan.as<ChanTask>(),
+                     g.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], (int)g.tiles.size(), stream);
+    return;
+  }
+  // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1;
+  // stage 3: A = Y, B = R panels (transposed), C = the Krylov output (FP64, row table)
+  if (reduced)
+    launch_grouped_gemm_mixed(precision, g.bm, g.bn, i == 0, i == 2, g.d_tiles.as<GemmTile>(),
+                              g.d_tasks.as<GemmTask>(), g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
+  else
+    launch_grouped_gemm_f64(g.bm, g.bn, cplx, g.d_tiles.as<GemmTile>(), g.d_tasks.as<GemmTask>(),
+                            g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
+}
+
+namespace {
+std::mutex g_tile_mu;
+std::unordered_map<std::string, std::pair<int, int>> g_tile_cache;   // signature -> (bm, bn)
+int g_tune_hits = 0, g_tune_misses = 0;
+
+bool autotune_enabled() {
+  static const bool on = [] {
+    const char *v = std::getenv("QLG_AUTOTUNE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+}  // namespace
+
+// Tile autotuner of the two grouped-GEMM stages.  Every tile shape (seven
+// real, four complex; the cost model's ranking only orders them) is timed on
+// this plan's own task lists and buffers (one warm-up and one timed launch each, HIP events
+// on the plan's stream); the fastest is kept.  Results are cached process-wide
+// under the stage's size-histogram signature, so sites with the same sector
+// profile reuse them.  The launches write only the plan's intermediates and
+// the output vector, which the matvec that follows overwrites.
+void Plan::autotune(const double *in, double *out) {
+  if (tuned_precision == precision) return;
+  tuned_precision = precision;
+  if (tiles_pinned || !autotune_enabled() || kind != PLAN_HEFF) return;
   const bool cplx = dtype == QLG_C128;
-  if (timing) {
+  const int variant = precision * 2 + (cplx ? 1 : 0);
+  hipStream_t stream = ctx->stream;
+  const Slots s = slots(in, out);
+  // tile lists are re-uploaded (and their buffers may grow) below: nothing
+  // queued earlier on the stream may still read them
+  QLG_HIP(hipStreamSynchronize(stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  QLG_HIP(hipEventCreate(&e0));
+  QLG_HIP(hipEventCreate(&e1));
+  const bool verbose = [] {
+    const char *v = std::getenv("QLG_VERBOSE");
+    return v && v[0] == '1';
+  }();
+  try {
+    for (int i : {0, 2}) {
+      Stage &g = st[i];
+      if (g.tasks.empty()) continue;
+      const std::string key = tile_signature(g, i, variant);
+      std::pair<int, int> chosen{0, 0};
+      {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = g_tile_cache.find(key);
+        if (it != g_tile_cache.end()) {
+          chosen = it->second;
+          ++g_tune_hits;
+        }
+      }
+      if (chosen.first == 0) {
+        const auto rank = rank_tile_shapes(g, cplx);
+        double best_ms = 1e300;
+        for (size_t c = 0; c < rank.size(); ++c) {
+          const auto sh = tile_shape(rank[c].second);
+          g.make_tiles(sh.first, sh.second);
+          g.upload(stream);
+          double ms_min = 1e300;
+          for (int rep = 0; rep < 2; ++rep) {
+            QLG_HIP(hipEventRecord(e0, stream));
+            launch_stage(i, s);
+            QLG_HIP(hipEventRecord(e1, stream));
+            QLG_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            QLG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) ms_min = std::min(ms_min, (double)ms);
+          }
+          if (verbose)
+            std::fprintf(stderr, "[qlg tune] stage %d %dx%d: %.4f ms (model %.4f ms)\n", i + 1, sh.first, sh.second,
+                         ms_min, rank[c].first * 1e3);
+          if (ms_min < best_ms) {
+            best_ms = ms_min;
+            chosen = sh;
+          }
+        }
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        g_tile_cache[key] = chosen;
+        ++g_tune_misses;
+      }
+      if (chosen.first != g.bm || chosen.second != g.bn || g.tiles.empty()) {
+        g.make_tiles(chosen.first, chosen.second);
+        g.upload(stream);
+      }
+      drop_graphs();   // captured launches name the old tile lists
+      if (i == 0) {   // fill T1 and Y with this matvec's data before stage 3 is timed
+        launch_stage(0, s);
+        launch_stage(1, s);
+      }
+    }
+    QLG_HIP(hipStreamSynchronize(stream));
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (verbose)
+    std::fprintf(stderr, "[qlg tune] tiles %dx%d / %dx%d (cache hits %d, misses %d)\n", st[0].bm, st[0].bn, st[2].bm,
+                 st[2].bn, g_tune_hits, g_tune_misses);
+}
+
+void Plan::matvec_fused(const double *in, double *out) {
+  const bool reduced = precision != QLG_PREC_F64;
+  if (reduced) prepare_reduced();
+  autotune(in, out);
+  const Slots s = slots(in, out);
+  hipStream_t stream = ctx->stream;
+  const bool timed = timing && !capturing;
+  if (timed) {
     for (auto &e : ev)
       if (!e) QLG_HIP(hipEventCreate(&e));
     QLG_HIP(hipEventRecord(ev[0], stream));
   }
-  // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1
-  Stage &a = st[0];
-  if (reduced)
-    launch_grouped_gemm_mixed(precision, a.bm, a.bn, true, false, a.d_tiles.as<GemmTile>(), a.d_tasks.as<GemmTask>(),
-                              a.d_segs.as<GemmSeg>(), s, (int)a.tiles.size(), stream);
-  else
-    launch_grouped_gemm_f64(a.bm, a.bn, cplx, a.d_tiles.as<GemmTile>(), a.d_tasks.as<GemmTask>(),
-                            a.d_segs.as<GemmSeg>(), s, (int)a.tiles.size(), stream);
-  if (timing) QLG_HIP(hipEventRecord(ev[1], stream));
-  Stage &b = st[1];
-  launch_mpo_stage(b.bm, b.bn, cplx, reduced, b.d_tiles.as<GemmTile>(), b.d_chan.as<ChanTask>(),
-                   b.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], (int)b.tiles.size(), stream);
-  if (timing) QLG_HIP(hipEventRecord(ev[2], stream));
-  // stage 3: A = Y, B = R panels (transposed), C = the Krylov output (FP64, row table)
-  Stage &c = st[2];
-  if (reduced)
-    launch_grouped_gemm_mixed(precision, c.bm, c.bn, false, true, c.d_tiles.as<GemmTile>(), c.d_tasks.as<GemmTask>(),
-                              c.d_segs.as<GemmSeg>(), s, (int)c.tiles.size(), stream);
-  else
-    launch_grouped_gemm_f64(c.bm, c.bn, cplx, c.d_tiles.as<GemmTile>(), c.d_tasks.as<GemmTask>(),
-                            c.d_segs.as<GemmSeg>(), s, (int)c.tiles.size(), stream);
-  if (timing) QLG_HIP(hipEventRecord(ev[3], stream));
+  for (int i = 0; i < 3; ++i) {
+    launch_stage(i, s);
+    if (timed) QLG_HIP(hipEventRecord(ev[i + 1], stream));
+  }
 }
 
 void Plan::matvec(const double *in, double *out) {

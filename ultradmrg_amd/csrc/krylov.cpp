@@ -12,7 +12,9 @@
 // a_m and b_{m-1} from device memory), so its norm arrives with a_m in the
 // same copy.  If the solver stops at m, that speculative residual is simply
 // discarded: the ground state only combines v_0..v_m.
+#include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "krylov.h"
@@ -171,6 +173,204 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
     }
     energy0 = e_new;
   }
+}
+
+// Batched Lanczos ground states of independent chains (a batch plan).  Every
+// chain follows the single-chain recurrence and stop rule above with its own
+// scalars; the chains advance in lock step, so each iteration is one grouped
+// matvec plus segmented vector kernels for all of them.  A chain that has
+// stopped gets a zero scale factor, so its segment of later basis vectors is
+// zero and costs nothing but the launch slots.  With graphs enabled, the
+// device work of iteration m is captured once as a hipGraph and replayed by
+// later calls on the same plan.
+void lanczos_gs_batch(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos_result *res) {
+  QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
+  QLG_CHECK(p.batch_off.size() >= 2, "batched Lanczos needs a batch plan");
+  QLG_CHECK(p.dtype == QLG_F64 && p.precision == QLG_PREC_F64, "batched Lanczos runs real FP64 chains");
+  Ctx &c = *p.ctx;
+  hipStream_t st = c.stream;
+  const int ns = (int)p.batch_off.size() - 1;
+  const int64_t maxit = prm.max_iterations;
+  Workspace w(p, maxit);
+  // chunk table
+  std::vector<SegChunk> chunks;
+  std::vector<int32_t> cbeg(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    cbeg[s] = (int32_t)chunks.size();
+    for (int64_t b = p.batch_off[s]; b < p.batch_off[s + 1]; b += kSegChunk)
+      chunks.push_back({b, std::min(b + kSegChunk, p.batch_off[s + 1]), s, 0});
+  }
+  cbeg[ns] = (int32_t)chunks.size();
+  const int nch = (int)chunks.size();
+  QLG_CHECK(nch > 0, "batched Lanczos: empty vectors");
+  // device workspace: chunks | chunk_begin | scalars ab[(maxit+1)][2 ns] | factors f[(maxit+1)][ns] |
+  // partials[nch] | coefficient table | basis pointers
+  const size_t o_chunks = 0, o_cbeg = o_chunks + sizeof(SegChunk) * nch;
+  const size_t o_ab = (o_cbeg + sizeof(int32_t) * (ns + 1) + 255) / 256 * 256;
+  const size_t o_f = o_ab + sizeof(double) * 2 * ns * (maxit + 1);
+  const size_t o_part = o_f + sizeof(double) * ns * (maxit + 1);
+  const size_t o_coef = o_part + sizeof(double) * nch;
+  const size_t o_ptr = o_coef + sizeof(double) * ns * (maxit + 1);
+  const size_t total = o_ptr + sizeof(double *) * (maxit + 1);
+  if (p.seg_ws.bytes < total || p.graph_maxit != maxit || p.graph_basis != w.slot0) p.drop_graphs();
+  p.seg_ws.ensure(total);
+  p.graph_maxit = maxit;
+  p.graph_basis = w.slot0;
+  char *ws = p.seg_ws.as<char>();
+  SegTable tab;
+  tab.chunks = reinterpret_cast<const SegChunk *>(ws + o_chunks);
+  tab.chunk_begin = reinterpret_cast<const int32_t *>(ws + o_cbeg);
+  tab.nchunks = nch;
+  tab.nseg = ns;
+  double *ab = reinterpret_cast<double *>(ws + o_ab);
+  double *fac = reinterpret_cast<double *>(ws + o_f);
+  double *part = reinterpret_cast<double *>(ws + o_part);
+  double *coefs = reinterpret_cast<double *>(ws + o_coef);
+  const double **ptrs = reinterpret_cast<const double **>(ws + o_ptr);
+  if (p.host_seg_n < (size_t)std::max<int64_t>(2 * ns, (maxit + 1) * ns)) {
+    if (p.host_seg) (void)hipHostFree(p.host_seg);
+    p.host_seg = nullptr;
+    p.host_seg_n = 0;
+    const size_t nh = (size_t)std::max<int64_t>(2 * ns, (maxit + 1) * ns);
+    QLG_HIP(hipHostMalloc((void **)&p.host_seg, nh * sizeof(double), hipHostMallocDefault));
+    p.host_seg_n = nh;
+  }
+  double *hs = p.host_seg;
+  QLG_HIP(hipMemcpyAsync(ws + o_chunks, chunks.data(), sizeof(SegChunk) * nch, hipMemcpyHostToDevice, st));
+  QLG_HIP(hipMemcpyAsync(ws + o_cbeg, cbeg.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, st));
+  auto fetch = [&](const double *dev, int nvals, std::vector<double> &out) {
+    QLG_HIP(hipMemcpyAsync(hs, dev, nvals * sizeof(double), hipMemcpyDeviceToHost, st));
+    QLG_HIP(hipStreamSynchronize(st));
+    out.assign(hs, hs + nvals);
+  };
+  auto upload_row = [&](double *dev, const std::vector<double> &v) {
+    for (int s = 0; s < ns; ++s) hs[s] = v[s];
+    QLG_HIP(hipMemcpyAsync(dev, hs, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    QLG_HIP(hipStreamSynchronize(st));   // hs is reused by the next fetch
+  };
+  // per-chain state
+  std::vector<std::vector<double>> a(ns, std::vector<double>(maxit + 1, 0.0)), b = a;
+  std::vector<double> energy0(ns), norm_gamma(ns), f(ns), h;
+  std::vector<int> active(ns, 1);
+  std::vector<std::vector<double>> fin(ns);
+  // v0 = psi / ||psi|| per chain
+  p.to_fused(psi, w.slot(0));
+  seg_dot(st, tab, w.slot(0), w.slot(0), part, ab, true);
+  fetch(ab, ns, h);
+  for (int s = 0; s < ns; ++s) {
+    QLG_CHECK(h[s] > 0.0, "initial state of chain " + std::to_string(s) + " has zero norm");
+    f[s] = 1.0 / h[s];
+  }
+  upload_row(fac, f);
+  seg_scal(st, tab, fac, w.slot(0));
+  p.matvec_fused(w.slot(0), w.slot(1));   // also settles the tile shapes before any capture
+  seg_dot(st, tab, w.slot(1), w.slot(0), part, ab, false);
+  seg_lanczos_residual(st, tab, w.slot(1), w.slot(0), ab, nullptr, nullptr, part, ab + ns);
+  fetch(ab, 2 * ns, h);
+  for (int s = 0; s < ns; ++s) {
+    a[s][0] = h[s];
+    norm_gamma[s] = h[ns + s];
+    energy0[s] = a[s][0];
+  }
+  auto step = [&](int64_t m) {
+    seg_scal(st, tab, fac + m * ns, w.slot(m));
+    p.matvec_fused(w.slot(m), w.slot(m + 1));
+    seg_dot(st, tab, w.slot(m + 1), w.slot(m), part, ab + m * 2 * ns, false);
+    seg_lanczos_residual(st, tab, w.slot(m + 1), w.slot(m), ab + m * 2 * ns, w.slot(m - 1),
+                         ab + (m - 1) * 2 * ns + ns, part, ab + m * 2 * ns + ns);
+  };
+  int64_t m = 0;
+  while (true) {
+    m += 1;
+    bool any = false;
+    for (int s = 0; s < ns; ++s) {
+      if (!active[s]) {
+        f[s] = 0.0;
+        continue;
+      }
+      if (norm_gamma[s] == 0.0) {   // invariant subspace
+        active[s] = 0;
+        f[s] = 0.0;
+        res[s].iters = m;
+        res[s].gs_eng = energy0[s];
+        if (m == 1) {
+          fin[s] = {1.0};
+        } else {
+          std::vector<double> vec(m);
+          double e;
+          if (!tridiag_gs(m, a[s].data(), b[s].data(), true, &e, vec.data()))
+            throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+          fin[s] = vec;
+        }
+        continue;
+      }
+      b[s][m - 1] = norm_gamma[s];
+      f[s] = 1.0 / norm_gamma[s];
+      any = true;
+    }
+    if (!any) break;
+    upload_row(fac + m * ns, f);
+    if (p.use_graphs) {
+      if ((int64_t)p.step_graphs.size() <= m) p.step_graphs.resize(m + 1, nullptr);
+      if (!p.step_graphs[m]) {
+        hipGraph_t g = nullptr;
+        QLG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        p.capturing = true;
+        try {
+          step(m);
+        } catch (...) {
+          p.capturing = false;
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        p.capturing = false;
+        QLG_HIP(hipStreamEndCapture(st, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        QLG_HIP(e);
+        p.step_graphs[m] = ex;
+      }
+      QLG_HIP(hipGraphLaunch(p.step_graphs[m], st));
+    } else {
+      step(m);
+    }
+    fetch(ab + m * 2 * ns, 2 * ns, h);
+    for (int s = 0; s < ns; ++s) {
+      if (!active[s]) continue;
+      a[s][m] = h[s];
+      norm_gamma[s] = h[ns + s];
+      double e_new;
+      if (!tridiag_gs(m + 1, a[s].data(), b[s].data(), false, &e_new, nullptr))
+        throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+      const int64_t dim = p.batch_off[s + 1] - p.batch_off[s];
+      if ((energy0[s] - e_new) < prm.error || m == dim || m == maxit - 1) {
+        std::vector<double> vec(m + 1);
+        if (!tridiag_gs(m + 1, a[s].data(), b[s].data(), true, &e_new, vec.data()))
+          throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+        fin[s] = vec;
+        active[s] = 0;
+        res[s].iters = m + 1;
+        res[s].gs_eng = e_new;
+      } else {
+        energy0[s] = e_new;
+      }
+    }
+  }
+  // ground states: per-chain combination of its own basis vectors
+  int64_t k = 0;
+  for (int s = 0; s < ns; ++s) k = std::max<int64_t>(k, (int64_t)fin[s].size());
+  for (int64_t j = 0; j < k; ++j)
+    for (int s = 0; s < ns; ++s) hs[j * ns + s] = j < (int64_t)fin[s].size() ? fin[s][j] : 0.0;
+  QLG_HIP(hipMemcpyAsync(coefs, hs, sizeof(double) * k * ns, hipMemcpyHostToDevice, st));
+  std::vector<const double *> bp(k);
+  for (int64_t j = 0; j < k; ++j) bp[j] = w.slot(j);
+  QLG_HIP(hipMemcpyAsync(ptrs, bp.data(), sizeof(double *) * k, hipMemcpyHostToDevice, st));
+  p.vtmp_out.ensure((size_t)w.nd * sizeof(double));
+  seg_lincomb(st, tab, (int)k, coefs, ptrs, p.vtmp_out.as<double>());
+  p.to_canonical(p.vtmp_out.as<double>(), psi);
+  QLG_HIP(hipStreamSynchronize(st));
 }
 
 void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &prm, int64_t &iters) {

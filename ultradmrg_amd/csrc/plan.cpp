@@ -242,20 +242,21 @@ void validate_stage(const Stage &s, const int64_t *slot_elems) {
   }
 }
 
-// Pick the FP64 tile shape minimising a simple cost model: MFMA time of the
-// padded tiles spread over 256 CUs (2 workgroups each), bounded below by the
-// longest single tile, plus a fixed per-tile cost.
-void autotune_tiles(Stage &s, bool cplx) {
-  static const int shapes[][2] = {{32, 32}, {64, 32}, {32, 64}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
+// Tile shapes of the grouped GEMMs ranked by a cost model: MFMA time of the
+// padded tiles at the FP64 peak, operand streaming at an L2-assisted rate,
+// the longest single tile, plus a fixed latency per tile.  Returns (estimate,
+// shape index) pairs, best first.
+static const int kShapes[][2] = {{32, 32}, {64, 32}, {32, 64}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
+
+std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx) {
   const int nshapes = cplx ? 4 : 7;   // complex tiles keep two accumulator planes: cap at 64x64
-  double best = 1e300;
-  int bm = 64, bn = 64;
   std::vector<int64_t> kpad(s.tasks.size(), 0);
   for (size_t t = 0; t < s.tasks.size(); ++t)
     for (int q = 0; q < s.tasks[t].seg_count; ++q) kpad[t] += (s.segs[s.tasks[t].seg_begin + q].K + 15) / 16 * 16;
   const double ebytes = cplx ? 16.0 : 8.0, fmul = cplx ? 4.0 : 1.0;
+  std::vector<std::pair<double, int>> out;
   for (int si = 0; si < nshapes; ++si) {
-    const int *sh = shapes[si];
+    const int *sh = kShapes[si];
     double flops = 0, bytes = 0, longest = 0, ntiles = 0;
     for (size_t t = 0; t < s.tasks.size(); ++t) {
       const GemmTask &k = s.tasks[t];
@@ -270,14 +271,47 @@ void autotune_tiles(Stage &s, bool cplx) {
     }
     // MFMA time at the FP64 peak, operand streaming at an L2-assisted 12 TB/s,
     // and ~1.5 us of exposed latency per tile spread over 1024 resident tiles
-    const double est = std::max({flops / 78.6e12, bytes / 12e12, longest}) + ntiles * 1.5e-6 / 1024.0;
-    if (est < best * 0.97) {
-      best = est;
-      bm = sh[0];
-      bn = sh[1];
-    }
+    out.push_back({std::max({flops / 78.6e12, bytes / 12e12, longest}) + ntiles * 1.5e-6 / 1024.0, si});
   }
-  s.make_tiles(bm, bn);
+  std::stable_sort(out.begin(), out.end());
+  return out;
+}
+
+void model_tiles(Stage &s, bool cplx) {
+  const auto r = rank_tile_shapes(s, cplx);
+  s.make_tiles(kShapes[r[0].second][0], kShapes[r[0].second][1]);
+}
+
+std::pair<int, int> tile_shape(int index) { return {kShapes[index][0], kShapes[index][1]}; }
+
+// Key of the tile autotuner's cache: the stage, the arithmetic variant, the
+// log2 of the stage's FLOPs and its FLOP-weighted histogram over
+// (log2 M, log2 N, log2 K) cells, each cell's share rounded to quarters.
+std::string tile_signature(const Stage &s, int stage, int variant) {
+  auto ilog2 = [](int64_t v) {
+    int r = 0;
+    while (v > 1) {
+      v >>= 1;
+      ++r;
+    }
+    return r;
+  };
+  std::map<int, double> cells;
+  double tot = 0;
+  for (const GemmTask &k : s.tasks) {
+    int64_t K = 0;
+    for (int q = 0; q < k.seg_count; ++q) K += s.segs[k.seg_begin + q].K;
+    const double f = 2.0 * k.M * k.N * K;
+    cells[ilog2(k.M) * 1024 + ilog2(k.N) * 32 + ilog2(K)] += f;
+    tot += f;
+  }
+  std::string key = std::to_string(stage) + "/" + std::to_string(variant) + "/" +
+                    std::to_string(tot > 0 ? (int)std::lround(std::log2(tot)) : -1);
+  for (const auto &c : cells) {
+    const long q = std::lround(4.0 * c.second / tot);
+    if (q > 0) key += ":" + std::to_string(c.first) + "x" + std::to_string(q);
+  }
+  return key;
 }
 
 // ---------------------------------------------------------------------------

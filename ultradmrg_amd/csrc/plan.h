@@ -66,6 +66,12 @@ struct Plan {
   // timing
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // tile autotuning of the two GEMM stages (per arithmetic variant)
+  bool tiles_pinned = false;
+  int tuned_precision = -1;
+  void autotune(const double *in, double *out);
+  void launch_stage(int i, const Slots &s);
+  Slots slots(const double *in, double *out);
   // Inside the library the Krylov vectors use the fused layout: for each left
   // sector a of the state, one row-major matrix [D_a x N_a] whose column
   // ranges are the state's blocks (a, ...).  pack_in / pack_out convert
@@ -82,6 +88,19 @@ struct Plan {
   bool reduced_ready = false;
   void prepare_reduced();
   DevBuf vtmp_in, vtmp_out;     // staging for qlg_matvec on canonical vectors
+  // batch plans: member m's vector occupies [batch_off[m], batch_off[m+1])
+  // of the batch vector (canonical and fused layouts alike)
+  std::vector<int64_t> batch_off;
+  // hipGraphs of the batched Lanczos steps (index m), captured on first use
+  bool use_graphs = false;
+  std::vector<hipGraphExec_t> step_graphs;
+  int64_t graph_maxit = 0;
+  const void *graph_basis = nullptr;
+  bool capturing = false;
+  void drop_graphs();
+  DevBuf seg_ws;                 // batched Lanczos: chunk table, per-chain scalars, partials
+  double *host_seg = nullptr;    // pinned per-chain scalars (2 x chains)
+  size_t host_seg_n = 0;
   std::vector<std::unique_ptr<DevBuf>> owned;   // operators the plan built itself (super-block form)
   ~Plan();
   void matvec(const double *in, double *out);        // canonical in / out
@@ -95,6 +114,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const
                  const qlg_bsten *psi, int precision);
 Plan *build_super_block_plan(Ctx *ctx, int64_t ngroups, const int64_t *nleft, const qlg_term *left,
                              const int64_t *nright, const qlg_term *right, const qlg_bsten *psi, int precision);
+Plan *build_batch_plan(Ctx *ctx, int64_t n, Plan *const *members);
 Plan *build_tensordot_plan(Ctx *ctx, const qlg_bsten *A, const qlg_bsten *B, int k, int conj_a, int conj_b);
 Plan *build_env_plan(Ctx *ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side);
 
@@ -110,7 +130,10 @@ void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64,
 void launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 void launch_block_copy(bool cplx, const CopyDesc *d, int n, const double *src, double *dst, hipStream_t st);
 void validate_stage(const Stage &s, const int64_t *slot_elems);
-void autotune_tiles(Stage &s, bool cplx);
+std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx);
+void model_tiles(Stage &s, bool cplx);
+std::pair<int, int> tile_shape(int index);
+std::string tile_signature(const Stage &s, int stage, int variant);
 
 void launch_grouped_gemm_f64(int bm, int bn, bool cplx, const GemmTile *tiles, const GemmTask *tasks,
                              const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st);

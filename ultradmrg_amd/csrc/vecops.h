@@ -16,4 +16,26 @@ void vec_lincomb(Ctx &c, int dtype, int64_t n, int k, const double *dev_coefs, c
 void vec_lanczos_residual(Ctx &c, int dtype, int64_t n, double *w, const double *v1, const double *a_dev,
                           const double *v0, const double *b_dev, double *dev_norm_out);
 
+// Segmented (batched-chain) forms, real data.  A SegTable lives in device
+// memory: chunks[nchunks] of at most kSegChunk elements, each inside one
+// segment, ordered by segment; chunk_begin[nseg + 1] indexes each segment's
+// chunks.  Per-segment coefficients and results are device arrays of nseg.
+constexpr int64_t kSegChunk = 8192;
+struct SegChunk {
+  int64_t begin, end;
+  int32_t seg, pad;
+};
+struct SegTable {
+  const SegChunk *chunks = nullptr;
+  const int32_t *chunk_begin = nullptr;
+  int nchunks = 0, nseg = 0;
+};
+void seg_dot(hipStream_t st, const SegTable &t, const double *x, const double *y, double *partials, double *out,
+             bool sqrt_result);
+void seg_scal(hipStream_t st, const SegTable &t, const double *alpha, double *x);
+void seg_lanczos_residual(hipStream_t st, const SegTable &t, double *w, const double *v1, const double *a,
+                          const double *v0, const double *b, double *partials, double *norm_out);
+void seg_lincomb(hipStream_t st, const SegTable &t, int k, const double *coefs, const double *const *bases,
+                 double *y);
+
 }  // namespace qlg

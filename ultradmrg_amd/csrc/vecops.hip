@@ -247,4 +247,97 @@ void vec_lanczos_residual(Ctx &c, int dtype, int64_t n, double *w, const double 
   QLG_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Segmented forms for batches of independent chains: the vector is a
+// concatenation of segments (one per chain), cut into chunks that never
+// straddle a segment.  Each workgroup handles one chunk; reductions write one
+// partial per chunk and a second kernel sums each segment's partials in chunk
+// order (deterministic).  Coefficients are per-segment device arrays.
+
+__global__ __launch_bounds__(VT) void seg_dot_kernel(const SegChunk *__restrict__ ch, const double *__restrict__ x,
+                                                     const double *__restrict__ y, double *__restrict__ partials) {
+  __shared__ double sh[1][VT / 64];
+  const SegChunk c = ch[blockIdx.x];
+  double acc[1] = {0.0};
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VT) acc[0] += x[i] * y[i];
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(VT) void seg_reduce_kernel(const int32_t *__restrict__ chunk_begin,
+                                                        const double *__restrict__ partials,
+                                                        double *__restrict__ out, int sqrt_result) {
+  __shared__ double sh[1][VT / 64];
+  const int s = blockIdx.x;
+  double acc[1] = {0.0};
+  for (int i = chunk_begin[s] + threadIdx.x; i < chunk_begin[s + 1]; i += VT) acc[0] += partials[i];
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) out[s] = sqrt_result ? sqrt(acc[0]) : acc[0];
+}
+
+__global__ __launch_bounds__(VT) void seg_scal_kernel(const SegChunk *__restrict__ ch,
+                                                      const double *__restrict__ alpha, double *__restrict__ x) {
+  const SegChunk c = ch[blockIdx.x];
+  const double a = alpha[c.seg];
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VT) x[i] *= a;
+}
+
+__global__ __launch_bounds__(VT) void seg_residual_kernel(const SegChunk *__restrict__ ch, double *__restrict__ w,
+                                                          const double *__restrict__ v1,
+                                                          const double *__restrict__ a,
+                                                          const double *__restrict__ v0,
+                                                          const double *__restrict__ b,
+                                                          double *__restrict__ partials) {
+  __shared__ double sh[1][VT / 64];
+  const SegChunk c = ch[blockIdx.x];
+  const double ac = a[c.seg], bc = v0 ? b[c.seg] : 0.0;
+  double acc[1] = {0.0};
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VT) {
+    double g = w[i] - ac * v1[i];
+    if (v0) g -= bc * v0[i];
+    w[i] = g;
+    acc[0] += g * g;
+  }
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(VT) void seg_lincomb_kernel(const SegChunk *__restrict__ ch, int k, int nseg,
+                                                         const double *__restrict__ coefs,
+                                                         const double *const *__restrict__ bases,
+                                                         double *__restrict__ y) {
+  const SegChunk c = ch[blockIdx.x];
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VT) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += coefs[(int64_t)j * nseg + c.seg] * bases[j][i];
+    y[i] = s;
+  }
+}
+
+void seg_dot(hipStream_t st, const SegTable &t, const double *x, const double *y, double *partials, double *out,
+             bool sqrt_result) {
+  hipLaunchKernelGGL(seg_dot_kernel, dim3(t.nchunks), dim3(VT), 0, st, t.chunks, x, y, partials);
+  hipLaunchKernelGGL(seg_reduce_kernel, dim3(t.nseg), dim3(VT), 0, st, t.chunk_begin, partials, out,
+                     (int)sqrt_result);
+  QLG_HIP(hipGetLastError());
+}
+
+void seg_scal(hipStream_t st, const SegTable &t, const double *alpha, double *x) {
+  hipLaunchKernelGGL(seg_scal_kernel, dim3(t.nchunks), dim3(VT), 0, st, t.chunks, alpha, x);
+  QLG_HIP(hipGetLastError());
+}
+
+void seg_lanczos_residual(hipStream_t st, const SegTable &t, double *w, const double *v1, const double *a,
+                          const double *v0, const double *b, double *partials, double *norm_out) {
+  hipLaunchKernelGGL(seg_residual_kernel, dim3(t.nchunks), dim3(VT), 0, st, t.chunks, w, v1, a, v0, b, partials);
+  hipLaunchKernelGGL(seg_reduce_kernel, dim3(t.nseg), dim3(VT), 0, st, t.chunk_begin, partials, norm_out, 1);
+  QLG_HIP(hipGetLastError());
+}
+
+void seg_lincomb(hipStream_t st, const SegTable &t, int k, const double *coefs, const double *const *bases,
+                 double *y) {
+  hipLaunchKernelGGL(seg_lincomb_kernel, dim3(t.nchunks), dim3(VT), 0, st, t.chunks, k, t.nseg, coefs, bases, y);
+  QLG_HIP(hipGetLastError());
+}
+
 }  // namespace qlg

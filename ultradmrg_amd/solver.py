@@ -217,6 +217,86 @@ class EffectiveHamiltonian:
             pass
 
 
+class BatchedEffectiveHamiltonian:
+    """Independent chains' effective Hamiltonians solved together
+    (``qlg_plan_batch``): each matvec stage of all members is one grouped
+    launch, and ``lanczos`` runs every member's Lanczos solve in lock step
+    (iteration counts and energies are those of separate solves).  With
+    ``graphs=True`` each Lanczos iteration is captured as a hipGraph on first
+    use and replayed by later solves on this batch."""
+
+    def __init__(self, members: Sequence[EffectiveHamiltonian], graphs: bool = False):
+        self.members = list(members)
+        self.ctx = self.members[0].ctx
+        self.lib = self.ctx.lib
+        n = len(self.members)
+        plans = (C.c_void_p * n)(*[m.h.value if isinstance(m.h, C.c_void_p) else m.h for m in self.members])
+        h = C.c_void_p()
+        torch.cuda.synchronize()
+        self.ctx.check(self.lib.qlg_plan_batch(self.ctx.h, n, plans, C.byref(h)))
+        self.h = h
+        cnt = C.c_int64()
+        offs = (C.c_int64 * (n + 1))()
+        self.ctx.check(self.lib.qlg_plan_batch_offsets(h, C.byref(cnt), offs))
+        self.offsets = list(offs)
+        self.size = self.offsets[-1]
+        self.device = self.members[0].device
+        self.dtype = self.members[0].dtype
+        if graphs:
+            self.ctx.check(self.lib.qlg_plan_use_graphs(h, 1))
+
+    def pack(self, states: Sequence[BlockTensor]) -> torch.Tensor:
+        buf = torch.zeros(max(self.size, 1), dtype=self.dtype, device=self.device)
+        for m, st, o in zip(self.members, states, self.offsets):
+            v = m.adopt(st)
+            buf[o:o + m.size] = v.data[:m.size]
+        return buf
+
+    def unpack(self, buf: torch.Tensor) -> List[BlockTensor]:
+        out = []
+        for m, o in zip(self.members, self.offsets):
+            out.append(BlockTensor(m.axes, m.layout_coords, buf[o:o + max(m.size, 1)].clone(), m.layout_offsets))
+        return out
+
+    def flops(self):
+        a, e = C.c_double(), C.c_double()
+        self.ctx.check(self.lib.qlg_plan_flops(self.h, C.byref(a), C.byref(e)))
+        return a.value, e.value
+
+    def enable_timing(self, on=True):
+        self.ctx.check(self.lib.qlg_plan_enable_timing(self.h, int(on)))
+
+    def stage_ms(self):
+        out = (C.c_double * 3)()
+        self.ctx.check(self.lib.qlg_plan_stage_ms(self.h, out))
+        return list(out)
+
+    def matvec_into(self, x: torch.Tensor, out: torch.Tensor, sync=True):
+        self.ctx.check(self.lib.qlg_matvec(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr())))
+        if sync:
+            self.ctx.synchronize()
+
+    def lanczos(self, init_states: Sequence[BlockTensor], params: LanczosParams) -> List[LanczosRes]:
+        work = self.pack(init_states)
+        torch.cuda.synchronize()
+        n = len(self.members)
+        res = (L.LanczosResult * n)()
+        prm = L.LanczosParams(params.error, params.max_iterations)
+        self.ctx.check(self.lib.qlg_lanczos_gs_batch(self.h, C.c_void_p(work.data_ptr()), C.byref(prm), res))
+        return [LanczosRes(int(r.iters), float(r.gs_eng), v) for r, v in zip(res, self.unpack(work))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qlg_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # reference-named entry points
 # ---------------------------------------------------------------------------

@@ -418,14 +418,12 @@ def _charge_counts(qns, nsites):
     return cnt
 
 
-def random_mps(m: Model, total, D: int, seed: int, device="cuda", dtype=torch.float64) -> List[BlockTensor]:
-    """Seeded random MPS in sector ``total`` (bond dimension <= D), right-canonical
-    from site 1, centre at site 0 (the same construction as the oracle's)."""
+def random_bonds(m: Model, total, D: int) -> List[Axis]:
+    """Virtual bond sectors of a random MPS in sector ``total``: every charge a
+    bond can carry, with degeneracy min(count from the left, count from the
+    right), scaled down to <= D in total."""
     qns, n = m.qns, m.n
     total = tuple(total)
-    rng = np.random.default_rng(seed)
-    pout = physical_axis(qns, OUT)
-    zero = tuple(0 for _ in qns[0])
     bonds = []
     for b in range(n - 1):
         cl, cr = _charge_counts(qns, b + 1), _charge_counts(qns, n - b - 1)
@@ -441,21 +439,50 @@ def random_mps(m: Model, total, D: int, seed: int, device="cuda", dtype=torch.fl
                 qm = max(degs, key=lambda q: degs[q])
                 degs[qm] -= 1
         bonds.append(Axis.make(sorted(degs.items()), OUT))
-    mps = []
+    return bonds
+
+
+def _site_axes(m: Model, total, bonds: List[Axis]):
+    """(axes, divergence) of every site tensor for the given bonds."""
+    pout = physical_axis(m.qns, OUT)
+    zero = tuple(0 for _ in m.qns[0])
     lvb = Axis.make([(zero, 1)], IN)
-    for i in range(n):
-        rvb = bonds[i] if i < n - 1 else Axis.make([(zero, 1)], OUT)
-        div = total if i == 0 else zero
-        axes = [lvb, pout, rvb]
+    out = []
+    for i in range(m.n):
+        rvb = bonds[i] if i < m.n - 1 else Axis.make([(zero, 1)], OUT)
+        out.append(([lvb, pout, rvb], tuple(total) if i == 0 else zero))
+        lvb = rvb.inverse()
+    return out
+
+
+def random_mps(m: Model, total, D: int, seed: int, device="cuda", dtype=torch.float64,
+               canonical: bool = True) -> List[BlockTensor]:
+    """Seeded random MPS in sector ``total`` (bond dimension <= D), right-canonical
+    from site 1, centre at site 0 (the same construction as the oracle's).
+    ``canonical=False`` returns the raw site tensors, filled with U[-1, 1) by a
+    seeded device generator (fast for long chains; not the oracle's stream)."""
+    n = m.n
+    rng = np.random.default_rng(seed)
+    gen = None if canonical else torch.Generator(device=device).manual_seed(seed)
+    mps = []
+    for axes, div in _site_axes(m, total, random_bonds(m, total, D)):
+        lvb, pout, rvb = axes
         blocks = {}
         for a in range(lvb.nsect):
             for s in range(pout.nsect):
                 for c in range(rvb.nsect):
                     if _flow(axes, (a, s, c)) == div:
                         shp = (lvb.degs[a], pout.degs[s], rvb.degs[c])
-                        blocks[(a, s, c)] = rng.uniform(-1, 1, size=shp)
-        mps.append(BlockTensor.from_blocks(axes, blocks, device=device, dtype=dtype))
-        lvb = rvb.inverse()
+                        blocks[(a, s, c)] = rng.uniform(-1, 1, size=shp) if canonical else shp
+        if canonical:
+            mps.append(BlockTensor.from_blocks(axes, blocks, device=device, dtype=dtype))
+        else:
+            keys = sorted(blocks)
+            size = sum(int(np.prod(blocks[k])) for k in keys)
+            data = torch.rand(max(size, 1), generator=gen, device=device, dtype=dtype) * 2 - 1
+            mps.append(BlockTensor(axes, keys, data))
+    if not canonical:
+        return mps
     for i in range(n - 1, 0, -1):
         r = block_svd(mps[i], 1, divergence(mps[i]), 0.0, 1, 10 ** 9)
         mps[i] = r.vt
