@@ -30,7 +30,10 @@ state, ``benchmark/ultradmrg_hubbard2d.cpp:130-248``) is ramped in FP64 through
 D = 10, 100, 500, 1000 (one sweep each) and three sweeps at D=2000, then two
 sweeps at D=2000 (Lanczos 1e-9 / 100, truncation 1e-8) are timed in FP64 and,
 from the same state, in the split-BF16 mode; energies are compared relative
-to FP64.
+to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
+random-field Heisenberg chains solved as one batch (one grouped launch per
+matvec stage, hipGraph-captured Lanczos iterations) with a loop over the
+chains.
 """
 from __future__ import annotations
 
@@ -125,6 +128,66 @@ def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 20
     return out
 
 
+def chains_comparison(n=32, seed=1004, reps=20):
+    """The batching form: n random-field Heisenberg chains (L ~ U[32, 128],
+    fields U(-2, 2), D in {128, 256, 512, 768}, seed 1004), the centre two-site
+    effective Hamiltonian of each.  Aggregate matvec TFLOP/s of the batch plan
+    (one grouped launch per stage for all chains) against a loop over the
+    chains' own plans, and the batched Lanczos solve (hipGraph-captured
+    iterations, replayed) against separate solves, same energies."""
+    import torch
+    from ultradmrg_amd.chains import random_field_chains
+    from ultradmrg_amd.solver import BatchedEffectiveHamiltonian, EffectiveHamiltonian, LanczosParams
+
+    def sync_time():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+
+    t0 = time.perf_counter()
+    chains = random_field_chains(n, seed)
+    hams = [EffectiveHamiltonian(c.eff_ham, c.psi) for c in chains]
+    build_s = time.perf_counter() - t0
+    xs = [h.adopt(c.psi) for h, c in zip(hams, chains)]
+    ys = [h.new_vector(zero=False) for h in hams]
+    flops = sum(h.flops()[0] for h in hams)
+    for h, x, y in zip(hams, xs, ys):          # settle tile shapes
+        h.matvec_into(x.data, y.data, sync=False)
+    t0 = sync_time()
+    for _ in range(reps):
+        for h, x, y in zip(hams, xs, ys):
+            h.matvec_into(x.data, y.data, sync=False)
+    loop_ms = (sync_time() - t0) * 1e3 / reps
+    batch = BatchedEffectiveHamiltonian(hams, graphs=True)
+    bx = batch.pack([c.psi for c in chains])
+    by = torch.empty_like(bx)
+    batch.matvec_into(bx, by)
+    t0 = sync_time()
+    for _ in range(reps):
+        batch.matvec_into(bx, by, sync=False)
+    batch_ms = (sync_time() - t0) * 1e3 / reps
+    prm = LanczosParams(1e-10, 100)
+    t0 = sync_time()
+    single = [h.lanczos(c.psi, prm) for h, c in zip(hams, chains)]
+    loop_lz = sync_time() - t0
+    batch.lanczos([c.psi for c in chains], prm)          # captures the iteration graphs
+    t0 = sync_time()
+    res = batch.lanczos([c.psi for c in chains], prm)    # replays them
+    batch_lz = sync_time() - t0
+    rel = max(abs(r.gs_eng - s.gs_eng) / abs(s.gs_eng) for r, s in zip(res, single))
+    same_iters = all(r.iters == s.iters for r, s in zip(res, single))
+    out = {"chains": n, "lengths": [c.length for c in chains], "D": [c.D for c in chains],
+           "build_s": build_s, "gflop_per_matvec": flops / 1e9,
+           "batched_matvec_ms": batch_ms, "batched_tflops": flops / (batch_ms * 1e-3) / 1e12,
+           "loop_matvec_ms": loop_ms, "loop_tflops": flops / (loop_ms * 1e-3) / 1e12,
+           "matvec_speedup": loop_ms / batch_ms,
+           "lanczos_batched_graph_s": batch_lz, "lanczos_loop_s": loop_lz, "lanczos_speedup": loop_lz / batch_lz,
+           "lanczos_iters": [r.iters for r in res], "same_iterations": same_iters, "max_rel_dE": rel}
+    batch.close()
+    for h in hams:
+        h.close()
+    return out
+
+
 def time_matvec(H, x, y, steps, warmup):
     H.ctx.synchronize()
     for _ in range(warmup):
@@ -165,6 +228,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--split", default="bf16x6", help="reduced-precision mode compared with FP64")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-chains", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the matvec is not sharded across GPUs in this build "
@@ -250,6 +314,8 @@ def main():
                         "issued_mfma_tflops": execf * mult / (ms_split * 1e-3) / 1e12,
                         "note": "algorithmic FLOPs per second; issued = executed FLOPs x products per term"}
     H.close()
+    if not a.no_chains:
+        out["batching"] = chains_comparison()
     if not a.no_sweep:
         out["sweep"] = hubbard_sweep_comparison(split=a.split)
     if not a.no_cpu:
