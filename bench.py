@@ -33,7 +33,8 @@ from the same state, in the split-BF16 mode; energies are compared relative
 to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
 random-field Heisenberg chains solved as one batch (one grouped launch per
 matvec stage, hipGraph-captured Lanczos iterations) with a loop over the
-chains.
+chains; unless ``--no-tdvp``, ``tdvp`` times three complex128 TDVP steps at
+D=1500 on the 4x8 Hubbard doublon quench.
 """
 from __future__ import annotations
 
@@ -126,6 +127,49 @@ def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 20
     out["speedup"] = sum(out["fp64"]["sweep_s"]) / sum(out[split]["sweep_s"])
     out["rel_dE"] = abs(es - e64) / abs(e64)
     return out
+
+
+def tdvp_measurement(lx=8, ly=4, D=1500, ramp=(64, 256, 512, 1000, 1500), steps=3, tau=0.05):
+    """The TDVP form on a bounded sample: the 4x8 Hubbard cylinder (t=1, U=8,
+    32 electrons, Sz=0) ramped in FP64 DMRG through ``ramp`` (one sweep each),
+    c^dag_up c^dag_dn applied on the centre site (the reference's cupccdnc),
+    then ``steps`` complex128 two-site TDVP steps (tau=0.05, Krylov dimension
+    20, D=1500).  Reports seconds per step with the expmv / SVD / environment
+    split, and the densities' sum (34 electrons) as a conservation check."""
+    import numpy as np
+    import torch
+    from ultradmrg_amd import mpo as M
+    from ultradmrg_amd import tdvp as T
+    from ultradmrg_amd.solver import LanczosParams
+    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, direct_state_mps
+
+    m = M.hubbard_cylinder(lx, ly, t=1.0, U=8.0)
+    n = m.n
+    labs = [1 if i % 2 == 0 else 2 for i in range(n)]
+    lp = LanczosParams(1e-8, 50)
+    mpo = M.build_mpo(m)
+    drv = TwoSiteVMPS(direct_state_mps(m, labs), mpo, SweepParams(1, ramp[0], ramp[0], 1e-8, lp, verbose=False))
+    t0 = time.perf_counter()
+    for Dr in ramp:
+        drv.p = SweepParams(1, Dr, Dr, 1e-8, lp, verbose=False)
+        drv.sweep()
+    ramp_s = time.perf_counter() - t0
+    e0 = drv.stats.log[-1]["e0"]
+    mps = [T.to_complex(t) for t in drv.mps]
+    site = n // 2
+    T.apply_local_op(mps, m.ops["cupccdnc"], site)
+    mps[0] = T.BlockTensor(mps[0].axes, mps[0].coords, mps[0].data / torch.linalg.vector_norm(mps[0].data),
+                           mps[0].offsets)
+    td = T.TwoSiteTDVP(mps, mpo, T.TDVPParams(tau, steps, 1, D, 1e-8, LanczosParams(1e-10, 20)))
+    for _ in range(steps):
+        td.sweep()
+    dens = td.measure(m.ops["nf"])
+    st = td.stats
+    return {"model": f"Hubbard {lx}x{ly} cylinder, U=8, t=1, half filling, doublon on site {site}",
+            "ramp": list(ramp), "ramp_s": ramp_s, "ground_energy": e0, "D": D, "tau": tau, "krylov_dim": 20,
+            "steps": steps, "step_s": st.step_seconds, "expmv_s": st.expmv_s, "svd_s": st.svd_s, "env_s": st.env_s,
+            "krylov_matvecs": st.matvecs, "density_sum": float(np.sum(dens)), "centre_density": float(dens[site]),
+            "dtype": "c128", "split_complex": None}
 
 
 def chains_comparison(n=32, seed=1004, reps=20):
@@ -229,6 +273,7 @@ def main():
     ap.add_argument("--split", default="bf16x6", help="reduced-precision mode compared with FP64")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-chains", action="store_true")
+    ap.add_argument("--no-tdvp", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the matvec is not sharded across GPUs in this build "
@@ -316,6 +361,8 @@ def main():
     H.close()
     if not a.no_chains:
         out["batching"] = chains_comparison()
+    if not a.no_tdvp:
+        out["tdvp"] = tdvp_measurement()
     if not a.no_sweep:
         out["sweep"] = hubbard_sweep_comparison(split=a.split)
     if not a.no_cpu:
