@@ -34,7 +34,9 @@ to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
 random-field Heisenberg chains solved as one batch (one grouped launch per
 matvec stage, hipGraph-captured Lanczos iterations) with a loop over the
 chains; unless ``--no-tdvp``, ``tdvp`` times three complex128 TDVP steps at
-D=1500 on the 4x8 Hubbard doublon quench.
+D=1500 on the 4x8 Hubbard doublon quench; unless ``--no-chain``,
+``heisenberg`` ramps the L=100 Heisenberg chain to D=1024 and times the
+mid-chain matvec.
 """
 from __future__ import annotations
 
@@ -126,6 +128,59 @@ def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 20
     out["split"] = split
     out["speedup"] = sum(out["fp64"]["sweep_s"]) / sum(out[split]["sweep_s"])
     out["rel_dE"] = abs(es - e64) / abs(e64)
+    return out
+
+
+def heisenberg_ramp_measurement(L=100, ramp=(256, 384, 512, 768, 1024, 1024), seed=1002, reps=30):
+    """The plain FP64 form on the Heisenberg chain: L=100, J=1, Sz=0, a seeded
+    random MPS (seed 1002) swept through the D ramp 256 -> 1024 (one sweep
+    each, Lanczos 1e-10 / 100, truncation 1e-10).  Then the centre is moved to
+    site 49 and the two-site matvec at sites (49, 50) is timed, as the
+    reference's profiler captures a mid-chain site
+    (``profiler/single_processor_lanczos_performance_test.cc:76-119``)."""
+    import torch
+    from ultradmrg_amd import mpo as M
+    from ultradmrg_amd.solver import EffectiveHamiltonian, LanczosParams
+    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, random_mps
+    from ultradmrg_amd.tensor import contract
+
+    m = M.heisenberg_chain(L)
+    lp = LanczosParams(1e-10, 100)
+    t0 = time.perf_counter()
+    drv = TwoSiteVMPS(random_mps(m, (0,), ramp[0], seed=seed), M.build_mpo(m),
+                      SweepParams(1, ramp[0], ramp[0], 1e-10, lp, verbose=False))
+    energies, secs = [], []
+    for Dr in ramp:
+        drv.p = SweepParams(1, Dr, Dr, 1e-10, lp, verbose=False)
+        energies.append(drv.sweep())
+        secs.append(drv.stats.sweep_seconds[-1])
+    total_s = time.perf_counter() - t0
+    mid = L // 2 - 1
+    for i in range(mid):
+        drv.update("r", i)
+    psi = contract(drv.mps[mid], drv.mps[mid + 1], 1)
+    H = EffectiveHamiltonian([drv.lenvs[mid], drv.mpo[mid], drv.mpo[mid + 1], drv.renvs[L - mid - 2]], psi)
+    x = H.adopt(psi)
+    y = H.new_vector(zero=False)
+    ms = time_matvec(H, x, y, reps, 3)
+    algo, _ = H.flops()
+    s_algo, _ = H.stage_work()
+    H.enable_timing(True)
+    acc = [0.0, 0.0, 0.0]
+    for _ in range(10):
+        H.matvec_into(x.data, y.data, sync=True)
+        acc = [p + q for p, q in zip(acc, H.stage_ms())]
+    H.enable_timing(False)
+    st_ms = [v / 10 for v in acc]
+    bond = drv.mps[mid].axes[2]
+    out = {"model": f"Heisenberg chain L={L}, J=1, Sz=0", "ramp": list(ramp), "sweep_s": secs,
+           "sweep_energies": energies, "total_s": total_s, "site": [mid, mid + 1], "bond_dim": sum(bond.degs),
+           "bond_sectors": bond.nsect, "largest_sector": max(bond.degs), "psi_elems": H.size,
+           "algo_gflop_per_matvec": algo / 1e9, "matvec_ms": ms, "matvec_tflops": algo / (ms * 1e-3) / 1e12,
+           "stage_ms": st_ms,
+           "stage_tflops": [f / (t * 1e-3) / 1e12 if t > 0 else None for f, t in zip(s_algo, st_ms)],
+           "frac_of_fp64_peak": algo / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    H.close()
     return out
 
 
@@ -274,6 +329,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-chains", action="store_true")
     ap.add_argument("--no-tdvp", action="store_true")
+    ap.add_argument("--no-chain", action="store_true", help="skip the L=100 Heisenberg ramp")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the matvec is not sharded across GPUs in this build "
@@ -359,6 +415,8 @@ def main():
                         "issued_mfma_tflops": execf * mult / (ms_split * 1e-3) / 1e12,
                         "note": "algorithmic FLOPs per second; issued = executed FLOPs x products per term"}
     H.close()
+    if not a.no_chain:
+        out["heisenberg"] = heisenberg_ramp_measurement()
     if not a.no_chains:
         out["batching"] = chains_comparison()
     if not a.no_tdvp:

@@ -297,3 +297,16 @@ def test_super_block_lanczos_matches_mpo_form():
     r_sb = LanczosSolver(_terms_to_device(mpo_form_to_terms(*eff)), to_device(psi), prm)
     assert abs(r_sb.gs_eng - r_mpo.gs_eng) <= 1e-12 * abs(r_mpo.gs_eng)
     assert abs(r_sb.iters - r_mpo.iters) <= 1
+
+
+@pytest.mark.parametrize("shape", ["32x32", "64x32", "32x64", "64x64", "128x64", "64x128", "128x128"])
+@pytest.mark.parametrize("kind,D,nh,seed", [("heisenberg", 300, 6, 2), ("hubbard", 160, 3, 4)])
+def test_matvec_every_tile_shape(monkeypatch, shape, kind, D, nh, seed):
+    """The tile autotuner may pick any shape for the two GEMM stages: each one,
+    pinned for both stages (QLG_TILES), matches the oracle."""
+    print(f"[case] matvec tile {shape} {kind} D={D}", flush=True)
+    monkeypatch.setenv("QLG_TILES", f"{shape},32x128,{shape}")
+    eff, psi = synthetic_eff_ham(kind, D, seed=seed, nsect_half=nh)
+    ref = eff_ham_mul_two_site_state(eff, psi)
+    _, out = gpu_matvec(eff, psi)
+    assert rel_err(to_oracle(out, ref), ref) < 1e-12

@@ -282,10 +282,17 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     flat[st.dst] = t.data
     per = {}   # bucket -> (E [B, b, b] descending eigenvectors, S [B, b], P = E^T X / S [B, b, mpad])
     svals = []
-    Xs, grams = {}, {}
+    Xs, grams, scales = {}, {}, {}
     for b, sis in st.buckets.items():
         mp, B = st.mpad[b], len(sis)
         X = flat[st.bucket_base[b]:st.bucket_base[b] + B * b * mp].view(B, b, mp)
+        # each sector matrix is scaled to unit Frobenius norm: the solver's
+        # tolerances are relative to the (padded) matrix norm, and the padding
+        # below must sit at the sector's own scale, not at that of the whole state
+        nrm = torch.linalg.vector_norm(X, dim=(1, 2))
+        nrm = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
+        X = X / nrm[:, None, None].to(dt)
+        scales[b] = nrm
         G = X @ X.mT.conj()
         ns = torch.tensor([st.sect[si]["n"] for si in sis], device=dev)
         pad = torch.arange(b, device=dev)[None, :] >= ns[:, None]
@@ -298,15 +305,38 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     for b in st.buckets:
         X = Xs[b]
         w, V = eig[b]
-        lam = w.flip(-1).clamp_min(0.0)
-        E = V.flip(-1)
-        S = lam.sqrt()
-        # Gram eigenvalues carry an absolute error ~ n eps lambda_max, i.e. singular
-        # values below ~1e-7 of the sector's largest are noise: they are set to
-        # zero and their singular vectors completed to an orthonormal set below
+        E = V.flip(-1)                                   # eigenvectors, descending eigenvalue
+        R = E.mT.conj() @ X                              # row j = s_j v_j^H (un-normalised)
+        # Singular values from the back-projected rows, s_j = |X^H e_j|: relative
+        # error ~ eps s_max / s_j, against eps (s_max / s_j)^2 for sqrt(lambda_j).
+        # Re-sorted descending within each sector (near-degenerate small values
+        # may swap).  Values at or below _SV_NOISE of the sector's largest count
+        # as zero; their singular vectors are completed to an orthonormal set below.
+        S = torch.linalg.vector_norm(R, dim=-1)
+        # padded positions (the last b - n eigenvectors) stay last
+        nsec = torch.tensor([st.sect[si]["n"] for si in st.buckets[b]], device=dev)
+        S = torch.where(torch.arange(S.shape[1], device=dev)[None, :] < nsec[:, None], S, torch.full_like(S, -1.0))
+        S, order = torch.sort(S, dim=-1, descending=True)
+        S = S.clamp_min(0.0)
+        E = torch.gather(E, 2, order[:, None, :].expand(E.shape))
+        R = torch.gather(R, 1, order[:, :, None].expand(R.shape))
         S = torch.where(S > _SV_NOISE * S[:, :1], S, torch.zeros_like(S))
         Sinv = torch.where(S > 0, 1.0 / torch.where(S > 0, S, torch.ones_like(S)), torch.zeros_like(S))
-        P = (E.mT.conj() @ X) * Sinv[..., None].to(dt)
+        P = R * Sinv[..., None].to(dt)                   # unit rows; zero rows for zero values
+        S = S * scales[b][:, None]                       # back to the tensor's scale
+        # Orthogonality between rows degrades as eps s_max^2 / (s_i s_j), ~1e-4
+        # at the noise threshold: Newton-Schulz steps toward the polar factor,
+        # P <- 1.5 P - 0.5 (P P^H) P, restore it quadratically.  The
+        # reconstruction moves by at most ~eps s_max^2 / s.  Zero rows stay zero.
+        mask = (S > 0).to(dt)
+        for _ in range(8):
+            PP = P @ P.mH
+            odev = float((PP - torch.diag_embed(mask)).abs().max())
+            if odev < 1e-13:
+                break
+            if odev > 0.5:
+                raise RuntimeError(f"block SVD: singular vectors too far from orthonormal ({odev:.2e})")
+            P = 1.5 * P - 0.5 * (PP @ P)
         per[b] = (E, S, P)
         svals.append(S)
     # one host transfer for the truncation decision
