@@ -42,13 +42,14 @@ extern "C" {
 #define QLG_F64 0   /* double                                            */
 #define QLG_C128 1  /* complex double, interleaved (re, im)              */
 
-/* Arithmetic of the sector GEMMs (real data).  The Krylov vectors, the
- * recurrence, the tridiagonal solve and all reductions stay FP64 in every
- * mode; the reduced modes keep the H_eff intermediates in FP32 and
- * accumulate in FP32.  qlg_lanczos_gs in a reduced mode reports the FP64
- * Rayleigh quotient of its ground state (one FP64 matvec), so the energy
+/* Arithmetic of the sector GEMMs.  The Krylov vectors, the recurrence, the
+ * tridiagonal solve (and the small-matrix exponential) and all reductions
+ * stay FP64 in every mode; the reduced modes keep the H_eff intermediates in
+ * FP32 and accumulate in FP32.  qlg_lanczos_gs in a reduced mode reports the
+ * FP64 Rayleigh quotient of its ground state (one FP64 matvec), so the energy
  * error is second order in the eigenvector error.  Complex plans support
- * QLG_PREC_F64 only. */
+ * QLG_PREC_F64, QLG_PREC_BF16X3 and QLG_PREC_BF16X6 (complex products from
+ * real split-piece products: Re.Re - Im.Im and Re.Im + Im.Re). */
 #define QLG_PREC_F64 0     /* FP64 MFMA                                          */
 #define QLG_PREC_F32 1     /* FP32 operands, FP32 MFMA                           */
 #define QLG_PREC_BF16X3 2  /* operands split in 2 bf16 terms (hi + lo), products
@@ -217,7 +218,8 @@ int qlg_linear_combine(qlg_ctx ctx, int dtype, int64_t n, int64_t k,
  * the normalised ground state on return.
  * qlg_lanczos_expmv: LanczosExpmvSolver,
  * include/qlmps/algorithm/tdvp/lanczos_expmv_solver_impl.h:70-184
- * (psi <- exp(-i*tau*H_eff) psi, norm of psi kept; QLG_C128 plans only).  */
+ * (psi <- exp(-i*tau*H_eff) psi, norm of psi kept; QLG_C128 plans only, in
+ * the plan's precision).  */
 int qlg_lanczos_gs(qlg_plan plan, void *psi_inout,
                    const qlg_lanczos_params *params, qlg_lanczos_result *res);
 int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau,

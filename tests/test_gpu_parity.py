@@ -310,3 +310,61 @@ def test_matvec_every_tile_shape(monkeypatch, shape, kind, D, nh, seed):
     ref = eff_ham_mul_two_site_state(eff, psi)
     _, out = gpu_matvec(eff, psi)
     assert rel_err(to_oracle(out, ref), ref) < 1e-12
+
+
+def _complexify(t: BSTensor, rng) -> BSTensor:
+    c = BSTensor(t.indexes, np.complex128, t.div())
+    c.blocks = {k: v + 1j * rng.uniform(-1, 1, size=v.shape) for k, v in t.blocks.items()}
+    return c
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("kind,D,nh,seed", [("heisenberg", 300, 6, 2), ("hubbard", 160, 3, 4)])
+def test_reduced_precision_complex_matvec(prec, kind, D, nh, seed):
+    """Split-BF16 sector GEMMs on complex data (the TDVP form) against the
+    complex128 oracle, to the same bars as the real modes."""
+    print(f"[case] reduced complex matvec {prec} {kind} D={D}", flush=True)
+    eff, psi = synthetic_eff_ham(kind, D, seed=seed, nsect_half=nh)
+    rng = np.random.default_rng(seed + 100)
+    L, Wl, Wr, R = eff
+    eff = [_complexify(L, rng), _complexify(Wl, np.random.default_rng(0)), _complexify(Wr, np.random.default_rng(1)),
+           _complexify(R, rng)]
+    for w in (eff[1], eff[2]):     # MPO tensors stay real-valued (complex dtype)
+        w.blocks = {k: v.real.astype(np.complex128) for k, v in w.blocks.items()}
+    psi = _complexify(psi, rng)
+    ref = eff_ham_mul_two_site_state(eff, psi)
+    H = EffectiveHamiltonian([to_device(t) for t in eff], to_device(psi), precision=prec)
+    out = H @ to_device(psi)
+    err = frob_rel(to_oracle(out, ref), ref)
+    print(f"  rel Frobenius error {err:.2e}", flush=True)
+    assert err < RED_TOL[prec]
+    H.set_precision("fp64")
+    assert rel_err(to_oracle(H @ to_device(psi), ref), ref) < 1e-12
+
+
+def test_complex_fp32_mode_is_refused():
+    from ultradmrg_amd._lib import QlgError
+    refkat.srand(0)
+    L, Wl, Wr, R, (Din, dout, Dout) = refkat.kat_eff_ham(10, 2, 2, complex_=True)
+    psi = random_fill(BSTensor([Din, dout, dout, Dout], np.complex128), (0,), np.random.default_rng(4), True)
+    with pytest.raises(QlgError, match="UNSUPPORTED"):
+        EffectiveHamiltonian([to_device(t) for t in (L, Wl, Wr, R)], to_device(psi), precision="fp32")
+
+
+def test_reduced_precision_complex_expmv():
+    """exp(-i tau H) psi with split-BF16 (6-term) matvecs against the dense exact
+    result (test_lanczos_expmv_solver.cc:221-315 operators): error < 1e-5."""
+    print("[case] reduced complex expmv", flush=True)
+    refkat.srand(0)
+    L, Wl, Wr, R, (Din, dout, Dout) = refkat.kat_eff_ham(10, 2, 2, complex_=True)
+    psi = random_fill(BSTensor([Din, dout, dout, Dout], np.complex128), (0,), np.random.default_rng(4), True)
+    H = EffectiveHamiltonian([to_device(t) for t in (L, Wl, Wr, R)], to_device(psi), precision="bf16x6")
+    v = psi.to_dense().reshape(-1)
+    Hd = _dense_two_site(L, Wl, Wr, R)
+    w, V = np.linalg.eigh(Hd)
+    exact = V @ (np.exp(-1j * 0.1 * w) * (V.conj().T @ v))
+    res = H.expmv(to_device(psi), 0.1, LanczosParams(1e-12, 100))
+    got = to_oracle(res.expmv, psi).to_dense().reshape(-1)
+    err = np.linalg.norm(got - exact) / np.linalg.norm(exact)
+    print(f"  rel error {err:.2e}", flush=True)
+    assert err < 1e-5

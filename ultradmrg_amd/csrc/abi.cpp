@@ -132,8 +132,8 @@ static void check_precision(int precision, int dtype) {
   QLG_CHECK(precision == QLG_PREC_F64 || precision == QLG_PREC_F32 || precision == QLG_PREC_BF16X3 ||
                 precision == QLG_PREC_BF16X6,
             "unknown precision");
-  if (precision != QLG_PREC_F64 && dtype != QLG_F64)
-    throw Error(QLG_ERR_UNSUPPORTED, "reduced-precision sector GEMMs are implemented for real data only");
+  if (precision == QLG_PREC_F32 && dtype != QLG_F64)
+    throw Error(QLG_ERR_UNSUPPORTED, "complex data runs FP64 or split-BF16 sector GEMMs (no FP32 form)");
 }
 
 static int make_plan(qlg_ctx ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
@@ -410,8 +410,6 @@ int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau, const qlg_lanc
   Plan &p = *plan->p;
   return guarded(p.ctx, [&] {
     QLG_CHECK(p.kind == PLAN_HEFF, "qlg_lanczos_expmv needs an effective-Hamiltonian plan");
-    if (p.precision != QLG_PREC_F64)
-      throw Error(QLG_ERR_UNSUPPORTED, "qlg_lanczos_expmv runs in FP64 (complex data)");
     lanczos_expmv(p, static_cast<double *>(psi_inout), tau, *params, *iters);
   });
 }

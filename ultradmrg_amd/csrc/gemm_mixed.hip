@@ -1,4 +1,5 @@
-// Reduced-precision grouped GEMMs for the real effective-Hamiltonian matvec.
+// Reduced-precision grouped GEMMs for the effective-Hamiltonian matvec
+// (real data; complex data in the split-BF16 modes).
 //
 // Same task / segment / tile lists as gemm_f64.hip.  A operands are FP32 in
 // memory (the plan keeps FP32 copies of the environment stacks, and the
@@ -354,7 +355,208 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmTile *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Split-BF16, complex data (stored (re, im)): the LDS slices hold real and
+// imaginary planes, and three FP32 accumulator planes collect
+// Re A . Re B, Im A . Im B and Re A . Im B + Im A . Re B, each from the same
+// split-piece products as the real kernel; Re C is the difference of the
+// first two, formed once at the end.
+template <int FM, int FN, typename TC>
+__device__ inline void store_tile_cplx(const GemmTask &tk, const GemmTile &tl, const Slots &slots, int wm0, int wn0,
+                                       int lane, const flt4 (&acc)[3][FM][FN]) {
+  int64_t crow[FM][4];
+  out_rows<FM>(tk, tl, slots, wm0, lane, crow);
+  TC *C = reinterpret_cast<TC *>(slots.p[tk.c_slot]) + 2 * tk.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = tl.n0 + wn0 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+        if (row < tk.M && col < tk.N) {
+          const int64_t ix = 2 * (crow[i][r] + col);
+          C[ix] = (TC)(acc[0][i][j][r] - acc[1][i][j][r]);
+          C[ix + 1] = (TC)acc[2][i][j][r];
+        }
+      }
+    }
+}
+
+template <typename T>
+__device__ inline float2 load_cplx_as_f32(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
+  if constexpr (sizeof(T) == 8) {
+    const double2 v = load_c128(r, elem == kOob ? kOob : elem * 16u);
+    return float2{(float)v.x, (float)v.y};
+  } else {
+    return load_c64(r, elem == kOob ? kOob : elem * 8u);
+  }
+}
+
+template <int BM, int BN, int NPC, typename TB, typename TC>
+__global__ __launch_bounds__(256) void gemm_bf16_cplx_kernel(const GemmTile *__restrict__ tiles,
+                                                             const GemmTask *__restrict__ tasks,
+                                                             const GemmSeg *__restrict__ segs, Slots slots,
+                                                             int ntiles) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int A_PER = BM * BKB / 256, B_PER = BKB * BN / 256;
+  constexpr int NPIECE = NPC == 3 ? 2 : 3;
+  __shared__ float As[2][2][BM][KP];
+  __shared__ float Bs[2][2][BN][KP];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  if ((int)blockIdx.x >= ntiles) return;
+  const GemmTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GemmTask tk = tasks[tl.task];
+  const int M = tk.M, N = tk.N;
+
+  flt4 acc[3][FM][FN];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[q][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+  float2 ra[A_PER], rb[B_PER];
+
+  auto b_kn = [&](int i, bool tb, int &k, int &n) {
+    const int e = tid + i * 256;
+    n = tb ? e / BKB : e % BN;
+    k = tb ? e % BKB : e / BN;
+  };
+  auto load = [&](const GemmSeg &sg, int k0) {
+    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + 2 * sg.a_off);
+    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + 2 * sg.b_off);
+    const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
+    const bool tb = sg.flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int e = tid + i * 256, m = e / BKB, k = e % BKB;
+      const uint32_t gm = tl.m0 + m, gk = k0 + k;
+      ra[i] = load_cplx_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      const uint32_t gn = tl.n0 + n, gk = k0 + k;
+      rb[i] = load_cplx_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+    }
+  };
+  auto store = [&](int buf, uint8_t flags) {
+    const bool tb = flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int e = tid + i * 256;
+      As[buf][0][e / BKB][e % BKB] = ra[i].x;
+      As[buf][1][e / BKB][e % BKB] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      Bs[buf][0][n][k] = rb[i].x;
+      Bs[buf][1][n][k] = rb[i].y;
+    }
+  };
+  auto prod = [&](const bf16x8 (&a)[NPIECE], const bf16x8 (&b)[NPIECE], flt4 c) {
+    if constexpr (NPC == 6) {
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], c, 0, 0, 0);
+    }
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
+    return c;
+  };
+  auto step = [&](int buf) {
+    bf16x8 ap[2][FM][NPIECE], bp[2][FN][NPIECE];
+    const int k8 = 8 * (lane >> 4);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        float x[8];
+        const float *src = &As[buf][p][wm0 + i * 16 + (lane & 15)][k8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = src[j];
+        split8<NPIECE>(x, ap[p][i]);
+      }
+#pragma unroll
+      for (int jn = 0; jn < FN; ++jn) {
+        float x[8];
+        const float *src = &Bs[buf][p][wn0 + jn * 16 + (lane & 15)][k8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = src[j];
+        split8<NPIECE>(x, bp[p][jn]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        acc[0][i][j] = prod(ap[0][i], bp[0][j], acc[0][i][j]);
+        acc[1][i][j] = prod(ap[1][i], bp[1][j], acc[1][i][j]);
+        acc[2][i][j] = prod(ap[1][i], bp[0][j], prod(ap[0][i], bp[1][j], acc[2][i][j]));
+      }
+  };
+
+  const int s_end = tk.seg_begin + tk.seg_count;
+  int s = tk.seg_begin;
+  while (s < s_end && segs[s].K <= 0) ++s;
+  if (s < s_end) {
+    GemmSeg cur = segs[s];
+    int k0 = 0, buf = 0;
+    load(cur, 0);
+    store(0, cur.flags);
+    __syncthreads();
+    while (true) {
+      int ns = s, nk = k0 + BKB;
+      GemmSeg nxt = cur;
+      if (nk >= cur.K) {
+        ++ns;
+        nk = 0;
+        while (ns < s_end && segs[ns].K <= 0) ++ns;
+        if (ns < s_end) nxt = segs[ns];
+      }
+      const bool more = ns < s_end;
+      if (more) load(nxt, nk);
+      step(buf);
+      if (!more) break;
+      store(buf ^ 1, nxt.flags);
+      __syncthreads();
+      buf ^= 1;
+      s = ns;
+      k0 = nk;
+      cur = nxt;
+    }
+  }
+  store_tile_cplx<FM, FN, TC>(tk, tl, slots, wm0, wn0, lane, acc);
+}
+
+// ---------------------------------------------------------------------------
 // launchers.  b_f64: B operand FP64 (else FP32); c_f64: output FP64 (else FP32).
+template <typename TB, typename TC>
+static void launch_mixed_cplx(int mode, int bm, int bn, const GemmTile *tiles, const GemmTask *tasks,
+                              const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st) {
+  dim3 grid(ntiles), block(256);
+#define QLG_BFC(M_, N_, P_)                                                                                   \
+  if (bm == M_ && bn == N_) {                                                                                 \
+    hipLaunchKernelGGL((gemm_bf16_cplx_kernel<M_, N_, P_, TB, TC>), grid, block, 0, st, tiles, tasks, segs,   \
+                       slots, ntiles);                                                                        \
+    return;                                                                                                   \
+  }
+  if (mode == QLG_PREC_BF16X3) {
+    QLG_BFC(32, 32, 3) QLG_BFC(64, 32, 3) QLG_BFC(32, 64, 3) QLG_BFC(64, 64, 3)
+  } else if (mode == QLG_PREC_BF16X6) {
+    QLG_BFC(32, 32, 6) QLG_BFC(64, 32, 6) QLG_BFC(32, 64, 6) QLG_BFC(64, 64, 6)
+  }
+#undef QLG_BFC
+  throw Error(QLG_ERR_UNSUPPORTED, "unsupported complex reduced-precision tile shape or mode");
+}
+
 template <typename TB, typename TC>
 static void launch_mixed_typed(int mode, int bm, int bn, const GemmTile *tiles, const GemmTask *tasks,
                                const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st) {
@@ -399,10 +601,20 @@ void launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st)
   QLG_HIP(hipGetLastError());
 }
 
-void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, const GemmTile *tiles,
+void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, bool cplx, const GemmTile *tiles,
                                const GemmTask *tasks, const GemmSeg *segs, const Slots &slots, int ntiles,
                                hipStream_t st) {
   if (ntiles <= 0) return;
+  if (cplx) {
+    if (b_f64 && !c_f64)
+      launch_mixed_cplx<double, float>(mode, bm, bn, tiles, tasks, segs, slots, ntiles, st);
+    else if (!b_f64 && c_f64)
+      launch_mixed_cplx<float, double>(mode, bm, bn, tiles, tasks, segs, slots, ntiles, st);
+    else
+      throw Error(QLG_ERR_UNSUPPORTED, "reduced-precision GEMM operand combination");
+    QLG_HIP(hipGetLastError());
+    return;
+  }
   if (b_f64 && !c_f64)
     launch_mixed_typed<double, float>(mode, bm, bn, tiles, tasks, segs, slots, ntiles, st);
   else if (!b_f64 && c_f64)

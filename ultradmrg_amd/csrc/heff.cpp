@@ -720,15 +720,15 @@ void Plan::to_canonical(const double *fused, double *canon) {
 
 void Plan::prepare_reduced() {
   if (reduced_ready) return;
-  QLG_CHECK(dtype == QLG_F64, "reduced-precision modes need real data");
-  Lint32.ensure(std::max<int64_t>(lint_elems, 1) * sizeof(float));
-  Rint32.ensure(std::max<int64_t>(rint_elems, 1) * sizeof(float));
-  C32.ensure(std::max<int64_t>(c_elems, 1) * sizeof(float));
-  Y32.ensure(std::max<int64_t>(y_elems, 1) * sizeof(float));
-  QLG_HIP(hipMemsetAsync(Y32.p, 0, std::max<int64_t>(y_elems, 1) * sizeof(float), ctx->stream));
-  launch_f64_to_f32(Lint.as<double>(), Lint32.as<float>(), lint_elems, ctx->stream);
-  launch_f64_to_f32(Rint.as<double>(), Rint32.as<float>(), rint_elems, ctx->stream);
-  launch_f64_to_f32(C.as<double>(), C32.as<float>(), c_elems, ctx->stream);
+  const int64_t ed = elem_doubles(dtype);   // complex data: (re, im) pairs of floats
+  Lint32.ensure(std::max<int64_t>(lint_elems * ed, 1) * sizeof(float));
+  Rint32.ensure(std::max<int64_t>(rint_elems * ed, 1) * sizeof(float));
+  C32.ensure(std::max<int64_t>(c_elems * ed, 1) * sizeof(float));
+  Y32.ensure(std::max<int64_t>(y_elems * ed, 1) * sizeof(float));
+  QLG_HIP(hipMemsetAsync(Y32.p, 0, std::max<int64_t>(y_elems * ed, 1) * sizeof(float), ctx->stream));
+  launch_f64_to_f32(Lint.as<double>(), Lint32.as<float>(), lint_elems * ed, ctx->stream);
+  launch_f64_to_f32(Rint.as<double>(), Rint32.as<float>(), rint_elems * ed, ctx->stream);
+  launch_f64_to_f32(C.as<double>(), C32.as<float>(), c_elems * ed, ctx->stream);
   reduced_ready = true;
 }
 
@@ -758,7 +758,7 @@ void Plan::launch_stage(int i, const Slots &s) {
   // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1;
   // stage 3: A = Y, B = R panels (transposed), C = the Krylov output (FP64, row table)
   if (reduced)
-    launch_grouped_gemm_mixed(precision, g.bm, g.bn, i == 0, i == 2, g.d_tiles.as<GemmTile>(),
+    launch_grouped_gemm_mixed(precision, g.bm, g.bn, i == 0, i == 2, cplx, g.d_tiles.as<GemmTile>(),
                               g.d_tasks.as<GemmTask>(), g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
   else
     launch_grouped_gemm_f64(g.bm, g.bn, cplx, g.d_tiles.as<GemmTile>(), g.d_tasks.as<GemmTask>(),

@@ -51,14 +51,13 @@ struct Mfma<float> {    // v_mfma_f32_16x16x4: D lane l, reg r -> row 4 (l / 16)
 }  // namespace
 
 // T: element type of C, X and Y (double, or float for the reduced-precision
-// modes); CPLX (complex double) only with T = double.
+// modes); CPLX: complex data of that element type, stored (re, im).
 template <int BM, int BN, bool CPLX, typename T>
 __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restrict__ tiles,
                                                         const ChanTask *__restrict__ tasks,
                                                         const RowDesc *__restrict__ rows, const T *__restrict__ C,
                                                         const T *__restrict__ X, T *__restrict__ Y,
                                                         int ntiles) {
-  static_assert(!CPLX || sizeof(T) == 8, "complex data is double");
   using MF = Mfma<T>;
   typedef typename MF::acc_t acc_t;
   constexpr int WN = BN / 4;                // columns per wave
@@ -110,9 +109,18 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
       const bool ok = gm < M && gk < K;
       const int64_t ix = ok ? (int64_t)gm * K + gk : 0;
       if constexpr (CPLX) {
-        const double2 v = reinterpret_cast<const double2 *>(Cm)[ix];
-        As[0][k][m] = ok ? v.x : 0.0;
-        As[1][k][m] = ok ? v.y : 0.0;
+        T vr, vi;
+        if constexpr (sizeof(T) == 8) {
+          const double2 v = reinterpret_cast<const double2 *>(Cm)[ix];
+          vr = v.x;
+          vi = v.y;
+        } else {
+          const float2 v = reinterpret_cast<const float2 *>(Cm)[ix];
+          vr = v.x;
+          vi = v.y;
+        }
+        As[0][k][m] = ok ? vr : T(0);
+        As[1][k][m] = ok ? vi : T(0);
       } else {
         As[0][k][m] = ok ? Cm[ix] : T(0);
       }
@@ -127,8 +135,12 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const uint32_t el = (kok && okj[j]) ? (uint32_t)((int)rd.off + apj[j] * (int)rd.rs + bj[j]) : kOob;
-        if constexpr (CPLX) {
+        if constexpr (CPLX && sizeof(T) == 8) {
           const double2 v = load_c128(xr, el == kOob ? kOob : el * 16u);
+          rb[0][st][j] = v.x;
+          rb[1][st][j] = v.y;
+        } else if constexpr (CPLX) {
+          const float2 v = load_c64(xr, el == kOob ? kOob : el * 8u);
           rb[0][st][j] = v.x;
           rb[1][st][j] = v.y;
         } else {
@@ -150,9 +162,9 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
         for (int j = 0; j < FN; ++j) {
           acc[0][i][j] = MF::run(af[0][i], rb[0][st][j], acc[0][i][j]);
           if constexpr (CPLX) {
-            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[1][i], rb[1][st][j], acc[0][i][j], 0, 0, 0);
-            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[0][i], rb[1][st][j], acc[1][i][j], 0, 0, 0);
-            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[1][i], rb[0][st][j], acc[1][i][j], 0, 0, 0);
+            acc[0][i][j] = MF::run(-af[1][i], rb[1][st][j], acc[0][i][j]);
+            acc[1][i][j] = MF::run(af[0][i], rb[1][st][j], acc[1][i][j]);
+            acc[1][i][j] = MF::run(af[1][i], rb[0][st][j], acc[1][i][j]);
           }
         }
     }
@@ -169,8 +181,10 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
         if (tl.m0 + lr < M) {
           const RowDesc rd = rout[lr];
           const int64_t ix = rd.off + (int64_t)apj[j] * rd.rs + bj[j];
-          if constexpr (CPLX)
+          if constexpr (CPLX && sizeof(T) == 8)
             reinterpret_cast<double2 *>(Y)[ix] = double2{acc[0][i][j][r], acc[1][i][j][r]};
+          else if constexpr (CPLX)
+            reinterpret_cast<float2 *>(Y)[ix] = float2{acc[0][i][j][r], acc[1][i][j][r]};
           else
             Y[ix] = acc[0][i][j][r];
         }
@@ -239,7 +253,10 @@ void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles
   float *Yf = static_cast<float *>(Y);
 #define QLG_MPO(M_, N_)                                                                                       \
   if (bm == M_ && bn == N_) {                                                                                 \
-    if (f32)                                                                                                  \
+    if (f32 && cplx)                                                                                          \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, float>), grid, block, 0, st, tiles, tasks, rows, Cf,  \
+                         Xf, Yf, ntiles);                                                                     \
+    else if (f32)                                                                                             \
       hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float>), grid, block, 0, st, tiles, tasks, rows, Cf, \
                          Xf, Yf, ntiles);                                                                     \
     else if (cplx)                                                                                            \

@@ -285,8 +285,9 @@ void model_tiles(Stage &s, bool cplx) {
 std::pair<int, int> tile_shape(int index) { return {kShapes[index][0], kShapes[index][1]}; }
 
 // Key of the tile autotuner's cache: the stage, the arithmetic variant, the
-// log2 of the stage's FLOPs and its FLOP-weighted histogram over
-// (log2 M, log2 N, log2 K) cells, each cell's share rounded to quarters.
+// log2 of the stage's FLOPs and the (log2 M, log2 N, log2 K) cell that holds
+// the largest share of them.  Coarse on purpose: the sites of a sweep at a
+// fixed bond dimension share a key, so each is tuned once.
 std::string tile_signature(const Stage &s, int stage, int variant) {
   auto ilog2 = [](int64_t v) {
     int r = 0;
@@ -305,13 +306,15 @@ std::string tile_signature(const Stage &s, int stage, int variant) {
     cells[ilog2(k.M) * 1024 + ilog2(k.N) * 32 + ilog2(K)] += f;
     tot += f;
   }
-  std::string key = std::to_string(stage) + "/" + std::to_string(variant) + "/" +
-                    std::to_string(tot > 0 ? (int)std::lround(std::log2(tot)) : -1);
-  for (const auto &c : cells) {
-    const long q = std::lround(4.0 * c.second / tot);
-    if (q > 0) key += ":" + std::to_string(c.first) + "x" + std::to_string(q);
-  }
-  return key;
+  int top = -1;
+  double best = -1;
+  for (const auto &c : cells)
+    if (c.second > best) {
+      best = c.second;
+      top = c.first;
+    }
+  return std::to_string(stage) + "/" + std::to_string(variant) + "/" +
+         std::to_string(tot > 0 ? (int)std::lround(std::log2(tot)) : -1) + "/" + std::to_string(top);
 }
 
 // ---------------------------------------------------------------------------

@@ -124,7 +124,7 @@ std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed);
 void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
 void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles, const ChanTask *tasks,
                       const RowDesc *rows, const void *C, const void *X, void *Y, int ntiles, hipStream_t st);
-void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, const GemmTile *tiles,
+void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, bool cplx, const GemmTile *tiles,
                                const GemmTask *tasks, const GemmSeg *segs, const Slots &slots, int ntiles,
                                hipStream_t st);
 void launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);

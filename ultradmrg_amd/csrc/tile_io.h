@@ -32,4 +32,12 @@ __device__ inline double2 load_c128(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
   return double2{__builtin_bit_cast(double, lo), __builtin_bit_cast(double, hi)};
 }
 
+// complex float (re, im): two 4-byte loads (kOob + 4 lies outside the range too)
+__device__ inline float2 load_c64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  float2 v;
+  v.x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+  v.y = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off + 4u, 0, 0));
+  return v;
+}
+
 }  // namespace qlg

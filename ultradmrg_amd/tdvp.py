@@ -19,6 +19,9 @@ Follows ``include/qlmps/algorithm/tdvp/two_site_update_finite_tdvp_impl.h``:
   moved to site 0.
 
 Data is complex128; the Krylov exponential, SVD and environments run on the GPU.
+The sector GEMMs of the Krylov exponentials can run in the split-BF16 modes
+(``TDVPParams.precision``); the Krylov recurrence, the small-matrix
+exponential, the SVD and the environments stay FP64.
 """
 from __future__ import annotations
 
@@ -44,6 +47,7 @@ class TDVPParams:
     trunc_err: float
     lancz: LanczosParams
     verbose: bool = False
+    precision: str = "fp64"   # sector GEMMs of the Krylov exponentials: "fp64", "bf16x6" or "bf16x3"
 
 
 @dataclass
@@ -89,7 +93,8 @@ class TwoSiteTDVP:
             l, r, llen, rlen = target - 1, target, target - 1, n - target - 1
         t0 = time.perf_counter()
         psi = contract(self.mps[l], self.mps[r], 1)
-        H = EffectiveHamiltonian([self.lenvs[llen], self.mpo[l], self.mpo[r], self.renvs[rlen]], psi)
+        H = EffectiveHamiltonian([self.lenvs[llen], self.mpo[l], self.mpo[r], self.renvs[rlen]], psi,
+                                 precision=p.precision)
         res = H.expmv(psi, step, p.lancz)
         H.close()
         t1 = time.perf_counter()
@@ -114,7 +119,7 @@ class TwoSiteTDVP:
         n, p = len(self.mps), self.p
         t0 = time.perf_counter()
         H = EffectiveHamiltonian([self.lenvs[target], self.mpo[target], self.renvs[n - target - 1]],
-                                 self.mps[target])
+                                 self.mps[target], precision=p.precision)
         res = H.expmv(self.mps[target], -step, p.lancz)
         H.close()
         self.mps[target] = res.expmv
