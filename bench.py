@@ -184,13 +184,15 @@ def heisenberg_ramp_measurement(L=100, ramp=(256, 384, 512, 768, 1024, 1024), se
     return out
 
 
-def tdvp_measurement(lx=8, ly=4, D=1500, ramp=(64, 256, 512, 1000, 1500), steps=3, tau=0.05):
+def tdvp_measurement(lx=8, ly=4, D=1500, ramp=(64, 256, 512, 1000, 1500), steps=3, tau=0.05, split="bf16x6"):
     """The TDVP form on a bounded sample: the 4x8 Hubbard cylinder (t=1, U=8,
     32 electrons, Sz=0) ramped in FP64 DMRG through ``ramp`` (one sweep each),
     c^dag_up c^dag_dn applied on the centre site (the reference's cupccdnc),
     then ``steps`` complex128 two-site TDVP steps (tau=0.05, Krylov dimension
-    20, D=1500).  Reports seconds per step with the expmv / SVD / environment
-    split, and the densities' sum (34 electrons) as a conservation check."""
+    20, D=1500), once with FP64 and once with split-BF16 sector GEMMs from the
+    same state.  Reports seconds per step with the expmv / SVD / environment
+    split, the densities' sum (34 electrons) as a conservation check, and the
+    largest local-density difference between the two forms."""
     import numpy as np
     import torch
     from ultradmrg_amd import mpo as M
@@ -215,16 +217,24 @@ def tdvp_measurement(lx=8, ly=4, D=1500, ramp=(64, 256, 512, 1000, 1500), steps=
     T.apply_local_op(mps, m.ops["cupccdnc"], site)
     mps[0] = T.BlockTensor(mps[0].axes, mps[0].coords, mps[0].data / torch.linalg.vector_norm(mps[0].data),
                            mps[0].offsets)
-    td = T.TwoSiteTDVP(mps, mpo, T.TDVPParams(tau, steps, 1, D, 1e-8, LanczosParams(1e-10, 20)))
-    for _ in range(steps):
-        td.sweep()
-    dens = td.measure(m.ops["nf"])
-    st = td.stats
-    return {"model": f"Hubbard {lx}x{ly} cylinder, U=8, t=1, half filling, doublon on site {site}",
-            "ramp": list(ramp), "ramp_s": ramp_s, "ground_energy": e0, "D": D, "tau": tau, "krylov_dim": 20,
-            "steps": steps, "step_s": st.step_seconds, "expmv_s": st.expmv_s, "svd_s": st.svd_s, "env_s": st.env_s,
-            "krylov_matvecs": st.matvecs, "density_sum": float(np.sum(dens)), "centre_density": float(dens[site]),
-            "dtype": "c128", "split_complex": None}
+    out = {"model": f"Hubbard {lx}x{ly} cylinder, U=8, t=1, half filling, doublon on site {site}",
+           "ramp": list(ramp), "ramp_s": ramp_s, "ground_energy": e0, "D": D, "tau": tau, "krylov_dim": 20,
+           "steps": steps, "dtype": "c128"}
+    dens = {}
+    for prec in ("fp64", split):
+        td = T.TwoSiteTDVP(list(mps), mpo, T.TDVPParams(tau, steps, 1, D, 1e-8, LanczosParams(1e-10, 20),
+                                                        precision=prec))
+        for _ in range(steps):
+            td.sweep()
+        dens[prec] = td.measure(m.ops["nf"])
+        st = td.stats
+        out[prec] = {"step_s": st.step_seconds, "expmv_s": st.expmv_s, "svd_s": st.svd_s, "env_s": st.env_s,
+                     "krylov_matvecs": st.matvecs, "density_sum": float(np.sum(dens[prec])),
+                     "centre_density": float(dens[prec][site])}
+    out["split"] = split
+    out["max_density_error"] = float(np.abs(dens[split] - dens["fp64"]).max())
+    out["step_speedup"] = sum(out["fp64"]["step_s"]) / sum(out[split]["step_s"])
+    return out
 
 
 def chains_comparison(n=32, seed=1004, reps=20):
@@ -420,7 +430,7 @@ def main():
     if not a.no_chains:
         out["batching"] = chains_comparison()
     if not a.no_tdvp:
-        out["tdvp"] = tdvp_measurement()
+        out["tdvp"] = tdvp_measurement(split=a.split)
     if not a.no_sweep:
         out["sweep"] = hubbard_sweep_comparison(split=a.split)
     if not a.no_cpu:
