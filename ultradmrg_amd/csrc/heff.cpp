@@ -13,6 +13,7 @@
 // The Krylov vectors inside the library use the fused layout; the boundary
 // keeps the canonical one (pack / unpack copies at the solver's entry / exit).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -27,8 +28,35 @@
 
 namespace qlg {
 
+namespace {
+// QLG_VERBOSE=2: host-side phase times of plan construction (stderr)
+struct PhaseTimer {
+  bool on;
+  std::chrono::steady_clock::time_point t0, last;
+  std::string line;
+  PhaseTimer() {
+    const char *v = std::getenv("QLG_VERBOSE");
+    on = v && v[0] == '2';
+    t0 = last = std::chrono::steady_clock::now();
+  }
+  void mark(const char *name) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    line += std::string(" ") + name + " " +
+            std::to_string(std::chrono::duration<double, std::milli>(now - last).count()).substr(0, 6);
+    last = now;
+  }
+  ~PhaseTimer() {
+    if (on)
+      std::fprintf(stderr, "[qlg plan ms]%s total %.3f\n", line.c_str(),
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
+};
+}  // namespace
+
 Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *const *Wd, const qlg_bsten *Rd,
                  const qlg_bsten *psid, int precision) {
+  PhaseTimer pt;
   QLG_CHECK(nphys == 1 || nphys == 2, "nphys must be 1 or 2");
   std::unique_ptr<Plan> P(new Plan());
   Plan &p = *P;
@@ -68,6 +96,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   const int rk = psi.rank;
   const int bax = rk - 1;  // right bond axis of psi
 
+  pt.mark("parse");
   // ---- indexes of operator blocks ------------------------------------------
   std::map<int, std::vector<int64_t>> L_by_ket;             // a -> L blocks
   for (size_t b = 0; b < L.blk.size(); ++b) L_by_ket[L.blk[b][2]].push_back(b);
@@ -82,6 +111,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     R_by_bra[R.blk[b][0]].push_back(b);
   }
 
+  pt.mark("index");
   // ---- symbolic reference schedule: closure of the block set + FLOPs --------
   auto symbolic = [&](const std::set<Coords> &in, double *flops) {
     std::set<Coords> out;
@@ -167,6 +197,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     for (int i = 0; i < 3; ++i) p.st[i].algo_flops = f3[i];
   }
 
+  pt.mark("symbolic");
   // ---- vector layout ----------------------------------------------------------
   p.vec.rank = rk;
   p.vec.ax = psi.ax;
@@ -213,6 +244,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     }
   }
 
+  pt.mark("layout");
   // ---- L stacks (ket-major) and stage 1 --------------------------------------------
   struct LStack {
     int64_t base = 0, rows = 0;
@@ -269,6 +301,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     p.x_elems = t1;
   }
 
+  pt.mark("stage1");
   // ---- R panels (bra-major) ------------------------------------------------------------
   struct RPanel {
     int64_t base = 0, K = 0;
@@ -295,6 +328,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     rpn[bp] = P;
   }
 
+  pt.mark("rpanels");
   // ---- Y panels (per bra sector) and stage 3 ---------------------------------------------
   struct YPanel {
     int64_t base = 0, rows = 0, K = 0;
@@ -357,6 +391,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     p.y_elems = ybase;
   }
 
+  pt.mark("stage3");
   // ---- stage 2: channels (a'-sector, b-sector) -------------------------------------------
   std::vector<double> Wdn[2];
   std::vector<int64_t> wdims[2];
@@ -426,6 +461,35 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
             }
       }
     }
+  }
+  // Combined two-site operator, independent of the channel:
+  // Wc[(w, s1, t0), (s2, t1, z)] = sum_y Wl[w, s1, t0, y] Wr[y, s2, t1, z],
+  // formed once from the nonzero entries of the (sparse) MPO tensors.
+  std::vector<double> Wc;
+  int64_t wc_cols = 0;
+  if (nphys == 2) {
+    const int64_t nw = wdims[0][0], ns0 = wdims[0][1], nt0 = wdims[0][2], ny = wdims[0][3];
+    const int64_t ns1 = wdims[1][1], nt1 = wdims[1][2], nz = wdims[1][3];
+    wc_cols = ns1 * nt1 * nz;
+    Wc.assign((size_t)nw * ns0 * nt0 * wc_cols * ed, 0.0);
+    std::vector<std::vector<int64_t>> wr_nz(ny);   // nonzero (s2, t1, z) columns of Wr per y
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t col = 0; col < wc_cols; ++col) {
+        const int64_t i1 = y * wc_cols + col;   // Wr is [y][s2][t1][z] row-major
+        if (Wdn[1][i1 * ed] != 0.0 || (ed == 2 && Wdn[1][i1 * ed + 1] != 0.0)) wr_nz[y].push_back(col);
+      }
+    for (int64_t row = 0; row < nw * ns0 * nt0; ++row)
+      for (int64_t y = 0; y < ny; ++y) {
+        const int64_t i0 = row * ny + y;          // Wl is [w][s1][t0][y] row-major
+        const double ar = Wdn[0][i0 * ed], ai = ed == 2 ? Wdn[0][i0 * ed + 1] : 0.0;
+        if (ar == 0.0 && ai == 0.0) continue;
+        for (int64_t col : wr_nz[y]) {
+          const int64_t i1 = y * wc_cols + col, o = row * wc_cols + col;
+          const double br = Wdn[1][i1 * ed], bi = ed == 2 ? Wdn[1][i1 * ed + 1] : 0.0;
+          Wc[o * ed] += ar * br - ai * bi;
+          if (ed == 2) Wc[o * ed + 1] += ar * bi + ai * br;
+        }
+      }
   }
   std::vector<double> Ch;
   int64_t coff = 0;
@@ -513,17 +577,11 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
           vr = Wdn[0][i * ed];
           if (ed == 2) vi = Wdn[0][i * ed + 1];
         } else {
-          for (int64_t y = 0; y < wdims[0][3]; ++y) {
-            const int64_t i0 = Widx(0, in.wg, in.sg0, tg0, y), i1 = Widx(1, y, in.sg1, tg1, zg);
-            if (ed == 1) {
-              vr += Wdn[0][i0] * Wdn[1][i1];
-            } else {
-              const double ar = Wdn[0][2 * i0], ai = Wdn[0][2 * i0 + 1];
-              const double br = Wdn[1][2 * i1], bi = Wdn[1][2 * i1 + 1];
-              vr += ar * br - ai * bi;
-              vi += ar * bi + ai * br;
-            }
-          }
+          const int64_t row = (in.wg * wdims[0][1] + in.sg0) * wdims[0][2] + tg0;
+          const int64_t col = (in.sg1 * wdims[1][2] + tg1) * wdims[1][3] + zg;
+          const int64_t i = row * wc_cols + col;
+          vr = Wc[i * ed];
+          if (ed == 2) vi = Wc[i * ed + 1];
         }
         const int64_t e = coff + (int64_t)r * K + q;
         Ch[e * ed] = vr;
@@ -536,6 +594,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   }
   p.c_elems = coff;
 
+  pt.mark("stage2");
   // ---- bounds of every task, then tiles and uploads --------------------------------
   {
     int64_t slot_elems[NSLOTS] = {0};
@@ -598,6 +657,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     }
     p.st[i].upload(ctx->stream);
   }
+  pt.mark("tiles");
   p.d_pack_in.ensure(std::max<size_t>(1, p.pack_in.size()) * sizeof(CopyDesc));
   p.d_pack_out.ensure(std::max<size_t>(1, p.pack_out.size()) * sizeof(CopyDesc));
   if (!p.pack_in.empty()) {
@@ -614,6 +674,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   p.Rint.ensure(std::max<int64_t>(rint_elems, 1) * eb);
   p.lint_elems = lint_elems;
   p.rint_elems = rint_elems;
+  pt.mark("alloc");
   // Y entries no channel writes stay zero for the lifetime of the plan
   QLG_HIP(hipMemsetAsync(p.Y.p, 0, std::max<int64_t>(p.y_elems, 1) * eb, ctx->stream));
   if (p.c_elems > 0)
@@ -632,6 +693,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
                       p.Rint.as<double>(), ctx->stream);
     QLG_HIP(hipStreamSynchronize(ctx->stream));
   }
+  pt.mark("repack+sync");
   {
     // algorithmic HBM bytes per launch: operands read once, results written once
     const double e8 = 8.0 * ed;

@@ -129,14 +129,26 @@ class EffectiveHamiltonian:
         return BlockTensor(self.axes, self.layout_coords, buf, self.layout_offsets)
 
     def adopt(self, state: BlockTensor) -> BlockTensor:
-        """Return ``state`` in the plan's Krylov layout (copying blocks if the layouts differ)."""
+        """Return ``state`` in the plan's Krylov layout (copying blocks if the layouts differ).
+        The copy is one device gather/scatter over all blocks."""
         if state.coords == self.layout_coords and state.offsets == self.layout_offsets:
             return state
         out = self.new_vector()
         pos = {c: o for c, o in zip(self.layout_coords, self.layout_offsets)}
-        for c, o in zip(state.coords, state.offsets):
-            n = state.block_size(c)
-            out.data[pos[c]:pos[c] + n] = state.data[o:o + n]
+        sizes = [state.block_size(c) for c in state.coords]
+        keep = [i for i, n in enumerate(sizes) if n > 0]
+        if not keep:
+            return out
+        dev = out.data.device
+        n = torch.tensor([sizes[i] for i in keep], dtype=torch.int64)
+        src0 = torch.tensor([state.offsets[i] for i in keep], dtype=torch.int64)
+        dst0 = torch.tensor([pos[state.coords[i]] for i in keep], dtype=torch.int64)
+        total = int(n.sum())
+        prefix = torch.cumsum(n, 0) - n
+        table = torch.stack([src0 - prefix, dst0 - prefix]).to(dev)          # per-block shift
+        blk = torch.repeat_interleave(torch.arange(len(keep), device=dev), n.to(dev), output_size=total)
+        k = torch.arange(total, device=dev)
+        out.data[table[1][blk] + k] = state.data[table[0][blk] + k]
         return out
 
     def flops(self):

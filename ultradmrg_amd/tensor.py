@@ -87,7 +87,10 @@ class BlockTensor:
         return tuple(ax.degs[s] for ax, s in zip(self.axes, c))
 
     def block_size(self, c):
-        return int(np.prod(self.block_shape(c)))
+        n = 1
+        for ax, s in zip(self.axes, c):
+            n *= ax.degs[s]
+        return n
 
     # ----- construction -------------------------------------------------------------
     @classmethod
