@@ -265,6 +265,33 @@ def _complete_rows(A: torch.Tensor, bad: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def _gather_runs(src: torch.Tensor, starts, sizes, dev) -> torch.Tensor:
+    """Concatenation of the runs src[starts[i] : starts[i] + sizes[i]] (one device gather)."""
+    n = torch.tensor(sizes, dtype=torch.int64)
+    total = int(n.sum())
+    if total == 0:
+        return torch.zeros(1, dtype=src.dtype, device=dev)
+    shift = (torch.tensor(starts, dtype=torch.int64) - (torch.cumsum(n, 0) - n)).to(dev)
+    blk = torch.repeat_interleave(torch.arange(len(sizes), device=dev), n.to(dev), output_size=total)
+    return src[shift[blk] + torch.arange(total, device=dev)]
+
+
+def _gather_cols(src: torch.Tensor, starts, rows, cols, ld, dev) -> torch.Tensor:
+    """Concatenation of the row-major [rows[i] x cols[i]] sub-matrices starting at
+    src[starts[i]] with leading dimension ld[i] (one device gather)."""
+    size = [r * c for r, c in zip(rows, cols)]
+    n = torch.tensor(size, dtype=torch.int64)
+    total = int(n.sum())
+    if total == 0:
+        return torch.zeros(1, dtype=src.dtype, device=dev)
+    T = torch.tensor([starts, cols, ld, list(np.cumsum(size) - np.array(size))], dtype=torch.int64).to(dev)
+    blk = torch.repeat_interleave(torch.arange(len(size), device=dev), n.to(dev), output_size=total)
+    loc = torch.arange(total, device=dev) - T[3][blk]
+    c = T[1][blk]
+    r = torch.div(loc, c, rounding_mode="floor")
+    return src[T[0][blk] + r * T[2][blk] + (loc - r * c)]
+
+
 def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dmax: int) -> SVDResult:
     """Block SVD with truncation, through FP64 eigen-decompositions of each
     sector matrix's Gram matrix on its shorter side, batched by size bucket.
@@ -370,7 +397,13 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     for b, sis in st.buckets.items():
         for j, si in enumerate(sis):
             slot[si] = (b, j)
-    up, sp, vp = {}, {}, {}
+    # Per kept sector, U (rows x k) and V^H (k x cols) are made contiguous in two
+    # flat buffers; the blocks of u and vt are then cut out of them by one
+    # device gather each (u blocks are row ranges of U, vt blocks column ranges
+    # of V^H), and s is scattered onto its diagonal blocks.
+    Ulist, Vlist, Slist = [], [], []
+    ubase, vbase, ks = {}, {}, {}
+    uoff = voff = 0
     for mi, si in enumerate(kept_si):
         sc, k = st.sect[si], int(keep[si])
         b, j = slot[si]
@@ -389,16 +422,33 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
                 U = _complete_rows(U.mT.contiguous(), bad).mT
             else:
                 Vh = _complete_rows(Vh.contiguous(), bad)
-        sp[(mi, mi)] = torch.diag(Sk).to(dt)
-        for lk in sc["rks"]:
-            shp = tuple(ax.degs[s] for ax, s in zip(left, lk)) + (k,)
-            up[lk + (mi,)] = U[sc["ro"][lk]:sc["ro"][lk] + sc["rows"][lk], :].reshape(shp)
-        for rk in sc["cks"]:
-            shp = (k,) + tuple(ax.degs[s] for ax, s in zip(right, rk))
-            vp[(mi,) + rk] = Vh[:, sc["co"][rk]:sc["co"][rk] + sc["cols"][rk]].reshape(shp)
-    u = from_pieces(list(left) + [mid_out], up, dt, dev)
-    s = from_pieces([mid_in, mid_out], sp, dt, dev)
-    vt = from_pieces([mid_in] + list(right), vp, dt, dev)
+        Ulist.append(U.reshape(-1))
+        Vlist.append(Vh.reshape(-1))
+        Slist.append(Sk)
+        ubase[si], vbase[si], ks[si] = uoff, voff, k
+        uoff += sc["R"] * k
+        voff += k * sc["C"]
+    Uflat = torch.cat(Ulist).to(dt)
+    Vflat = torch.cat(Vlist).to(dt)
+    # u blocks (lk, mi): rows [ro, ro + rows) of U -> one contiguous run each
+    ublk = sorted((lk + (mi,), si, lk) for mi, si in enumerate(kept_si) for lk in st.sect[si]["rks"])
+    u_start = [ubase[si] + st.sect[si]["ro"][lk] * ks[si] for _, si, lk in ublk]
+    u_size = [st.sect[si]["rows"][lk] * ks[si] for _, si, lk in ublk]
+    u = BlockTensor(list(left) + [mid_out], [c for c, _, _ in ublk],
+                    _gather_runs(Uflat, u_start, u_size, dev))
+    # vt blocks (mi, rk): columns [co, co + cols) of V^H (k x C)
+    vblk = sorted(((mi,) + rk, si, rk) for mi, si in enumerate(kept_si) for rk in st.sect[si]["cks"])
+    vt = BlockTensor([mid_in] + list(right), [c for c, _, _ in vblk],
+                     _gather_cols(Vflat, [vbase[si] + st.sect[si]["co"][rk] for _, si, rk in vblk],
+                                  [ks[si] for _, si, _ in vblk], [st.sect[si]["cols"][rk] for _, si, rk in vblk],
+                                  [st.sect[si]["C"] for _, si, _ in vblk], dev))
+    # s: diagonal k x k blocks in sector order
+    kk = [ks[si] for si in kept_si]
+    sdata = torch.zeros(max(sum(k * k for k in kk), 1), dtype=dt, device=dev)
+    sb = np.concatenate([[0], np.cumsum([k * k for k in kk])[:-1]]).astype(np.int64)
+    diag_pos = torch.from_numpy(np.concatenate([sb[i] + np.arange(k) * (k + 1) for i, k in enumerate(kk)])).to(dev)
+    sdata[diag_pos] = torch.cat(Slist).to(dt)
+    s = BlockTensor([mid_in, mid_out], [(mi, mi) for mi in range(len(kept_si))], sdata)
     pr = np.sort(sv)[::-1][:D] ** 2
     pr = pr / pr.sum()
     pr = pr[pr > 0]
