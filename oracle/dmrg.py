@@ -21,16 +21,14 @@ reference; end-to-end energies pin the sweep.
 """
 from __future__ import annotations
 
-import itertools
 from dataclasses import dataclass
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, List, Sequence
 
 import numpy as np
 
-from .bsten import IN, OUT, BSTensor, Contract, ContractRotated, Dag, Index, qn_add, qn_neg, qn_sub
-from .lanczos import (LanczosParams, LanczosExpmvSolver, LanczosSolver, eff_ham_mul_single_site_state,
-                      eff_ham_mul_two_site_state)
-from .models import ModelSpec, build_mpo, phys_index
+from .bsten import IN, OUT, BSTensor, Contract, ContractRotated, Dag, Index, qn_add, qn_sub
+from .lanczos import LanczosParams, LanczosSolver, eff_ham_mul_two_site_state
+from .models import ModelSpec, phys_index
 
 
 # --------------------------------------------------------------------------

@@ -30,7 +30,7 @@ from typing import Callable, List, Sequence
 import numpy as np
 import scipy.linalg as sla
 
-from .bsten import BSTensor, Contract, ContractRotated, Dag, LinearCombine, vdot
+from .bsten import BSTensor, Contract, ContractRotated, LinearCombine, vdot
 
 
 @dataclass

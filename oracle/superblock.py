@@ -23,11 +23,11 @@ by the reference's Lanczos KAT and energies) on identical operators.
 """
 from __future__ import annotations
 
-from typing import List, Sequence, Tuple
+from typing import List, Tuple
 
 import numpy as np
 
-from .bsten import BSTensor, ContractRotated, Index, qn_sub, qn_add
+from .bsten import BSTensor, ContractRotated, Index
 
 Terms = List[Tuple[List[Tuple[BSTensor, BSTensor]], List[Tuple[BSTensor, BSTensor]]]]
 

@@ -32,7 +32,7 @@ from typing import List
 import numpy as np
 
 from .bsten import BSTensor, Contract, Index
-from .dmrg import UpdateSiteLenvs, UpdateSiteRenvs, block_svd, init_envs, trivial_env
+from .dmrg import UpdateSiteLenvs, UpdateSiteRenvs, block_svd, trivial_env
 from .lanczos import (LanczosExpmvSolver, LanczosParams, eff_ham_mul_single_site_state,
                       eff_ham_mul_two_site_state)
 

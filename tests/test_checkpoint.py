@@ -10,7 +10,6 @@ import json
 import os
 import signal
 
-import numpy as np
 import pytest
 import torch
 

@@ -6,7 +6,6 @@ result must be the one a separate solve gives: the same iteration count,
 energy within 1e-12 relative and the same ground state; the batched matvec
 equals each member's own matvec to 1e-13; with hipGraph capture enabled, a
 second solve replays the captured iterations and reproduces the first."""
-import numpy as np
 import pytest
 import torch
 

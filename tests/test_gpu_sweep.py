@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from oracle.bsten import BSTensor, Contract
-from oracle.dmrg import UpdateSiteLenvs, UpdateSiteRenvs, block_svd as o_block_svd, init_envs, random_mps as o_random_mps
+from oracle.dmrg import UpdateSiteLenvs, block_svd as o_block_svd, init_envs, random_mps as o_random_mps
 from oracle.models import build_mpo as o_build_mpo, heisenberg_chain as o_heis, hubbard_cylinder as o_hub
 from tests.convert import to_device, to_oracle
 from tests.workloads import synthetic_eff_ham

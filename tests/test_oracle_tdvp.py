@@ -8,7 +8,7 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from oracle.bsten import BSTensor, Contract
+from oracle.bsten import BSTensor
 from oracle.dmrg import init_envs, random_mps
 from oracle.lanczos import LanczosParams
 from oracle.models import build_mpo, dense_hamiltonian_sparse, heisenberg_chain, hubbard_cylinder

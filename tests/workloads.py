@@ -7,7 +7,7 @@ allowed block is present.
 """
 import numpy as np
 
-from oracle.bsten import IN, OUT, BSTensor, Index, qn_add, qn_sub
+from oracle.bsten import OUT, BSTensor, Index
 from oracle.models import build_mpo, heisenberg_chain, hubbard_cylinder, phys_index
 
 

@@ -42,15 +42,6 @@ def block_view(t: BlockTensor, i: int) -> torch.Tensor:
     return t.data[t.offsets[i]:t.offsets[i] + t.block_size(c)].view(t.block_shape(c))
 
 
-def from_pieces(axes, pieces: Dict[Tuple[int, ...], torch.Tensor], dtype, device) -> BlockTensor:
-    keys = sorted(pieces)
-    if keys:
-        flat = torch.cat([pieces[k].reshape(-1) for k in keys]).to(dtype)
-    else:
-        flat = torch.zeros(1, dtype=dtype, device=device)
-    return BlockTensor(axes, keys, flat)
-
-
 # ---------------------------------------------------------------------------
 # block SVD with truncation
 # ---------------------------------------------------------------------------

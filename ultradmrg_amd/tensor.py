@@ -11,9 +11,8 @@ descriptor handed to the library is zero-copy.
 from __future__ import annotations
 
 import ctypes as C
-import itertools
 from dataclasses import dataclass
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
