@@ -94,6 +94,19 @@ int qlg_ctx_destroy(qlg_ctx ctx);
 const char *qlg_last_error(qlg_ctx ctx);
 int qlg_ctx_synchronize(qlg_ctx ctx);
 
+/* Block cache of the library's own device buffers (plan workspaces, Krylov
+ * bases).  No counterpart in the reference, whose tensors allocate through
+ * qlten; here plans are built and destroyed at every site of a sweep, and the
+ * cache keeps their buffers for the next plan instead of a hipMalloc / hipFree
+ * pair each time.  The cache holds at most min(16 GiB, 1/8 of the device)
+ * (QLG_POOL_MB overrides; QLG_POOL=0 at context creation turns it off).
+ * qlg_ctx_release_cached returns the cached blocks to the driver (the blocks
+ * of live plans stay with them).  qlg_ctx_memory_stats writes
+ * {cached bytes, hipMalloc calls, reuses, hipFree calls}; all zero when the
+ * cache is off. */
+int qlg_ctx_release_cached(qlg_ctx ctx);
+int qlg_ctx_memory_stats(qlg_ctx ctx, int64_t stats[4]);
+
 /* ---- device memory plumbing (for hosts without their own HIP allocator) --
  * Allocations are plain hipMalloc memory on the context's device; copies run
  * on the context stream and return after completion.                      */

@@ -193,6 +193,42 @@ def test_complex_matvec_and_expmv_kat():
     assert res.iters == ores.iters
 
 
+def test_device_block_cache_reuse():
+    """Plans built and destroyed in turn, as a sweep does, take their device
+    buffers from the context's block cache: a plan of a shape seen before
+    is served without a hipMalloc and a smaller one reuses blocks too; the
+    results equal the first plan's bit for bit and the oracle's to 1e-12;
+    releasing the cache empties it."""
+    import os
+    if os.environ.get("QLG_POOL", "1")[:1] == "0":
+        pytest.skip("block cache switched off (QLG_POOL=0)")
+    print("[case] device block cache", flush=True)
+    ctx = T.default_context()
+    ctx.release_cached()
+    assert ctx.memory_stats()["cached_bytes"] == 0
+    eff, psi = synthetic_eff_ham("hubbard", 160, seed=4, nsect_half=3)
+    ref = eff_ham_mul_two_site_state(eff, psi)
+    gpu_matvec(eff, psi)[0].close()   # the first plan of this shape also runs the tile autotuner
+    H, out1 = gpu_matvec(eff, psi)
+    H.close()
+    s1 = ctx.memory_stats()
+    assert s1["cached_bytes"] > 0
+    H, out2 = gpu_matvec(eff, psi)
+    H.close()
+    s2 = ctx.memory_stats()
+    assert s2["mallocs"] == s1["mallocs"] and s2["reuses"] > s1["reuses"]
+    assert torch.equal(out1.data, out2.data)
+    assert rel_err(to_oracle(out2, ref), ref) < 1e-12
+    eff_s, psi_s = synthetic_eff_ham("hubbard", 96, seed=5, nsect_half=3)
+    ref_s = eff_ham_mul_two_site_state(eff_s, psi_s)
+    H, out3 = gpu_matvec(eff_s, psi_s)
+    H.close()
+    assert ctx.memory_stats()["reuses"] > s2["reuses"]
+    assert rel_err(to_oracle(out3, ref_s), ref_s) < 1e-12
+    ctx.release_cached()
+    assert ctx.memory_stats()["cached_bytes"] == 0
+
+
 def test_matvec_linearity_at_bench_size():
     """Size-independent property at the benchmark's D=2000 Hubbard shape:
     H(a x + b y) = a Hx + b Hy, and repeated matvecs are bit-identical."""
@@ -299,7 +335,8 @@ def test_super_block_lanczos_matches_mpo_form():
     assert abs(r_sb.iters - r_mpo.iters) <= 1
 
 
-@pytest.mark.parametrize("shape", ["32x32", "64x32", "32x64", "64x64", "128x64", "64x128", "128x128"])
+@pytest.mark.parametrize("shape", ["32x32", "64x32", "32x64", "64x64", "128x64", "64x128", "128x128", "64x16",
+                                   "128x16"])
 @pytest.mark.parametrize("kind,D,nh,seed", [("heisenberg", 300, 6, 2), ("hubbard", 160, 3, 4)])
 def test_matvec_every_tile_shape(monkeypatch, shape, kind, D, nh, seed):
     """The tile autotuner may pick any shape for the two GEMM stages: each one,

@@ -25,6 +25,7 @@ ERRORS = {1: "QLG_ERR_INVALID", 2: "QLG_ERR_HIP", 3: "QLG_ERR_ALLOC", 4: "QLG_ER
 # every entry point declared in include/qlgpu.h
 EXPORTS = [
     "qlg_ctx_create", "qlg_ctx_destroy", "qlg_last_error", "qlg_ctx_synchronize",
+    "qlg_ctx_release_cached", "qlg_ctx_memory_stats",
     "qlg_device_alloc", "qlg_device_free", "qlg_memcpy_h2d", "qlg_memcpy_d2h", "qlg_memcpy_d2d",
     "qlg_plan_two_site", "qlg_plan_one_site", "qlg_plan_super_block", "qlg_plan_destroy", "qlg_plan_set_precision",
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
@@ -84,6 +85,8 @@ def lib():
         "qlg_ctx_destroy": (C.c_int, [vp]),
         "qlg_last_error": (C.c_char_p, [vp]),
         "qlg_ctx_synchronize": (C.c_int, [vp]),
+        "qlg_ctx_release_cached": (C.c_int, [vp]),
+        "qlg_ctx_memory_stats": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "qlg_device_alloc": (C.c_int, [vp, i64, C.POINTER(vp)]),
         "qlg_device_free": (C.c_int, [vp, vp]),
         "qlg_memcpy_h2d": (C.c_int, [vp, vp, vp, i64]),

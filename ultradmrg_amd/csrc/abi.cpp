@@ -1,6 +1,7 @@
 // extern "C" entry points of libqlgpu (declared in include/qlgpu.h).  Every
 // call converts library exceptions into status codes and records the message
 // on the context.
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -22,9 +23,17 @@ struct qlg_plan_s {
 namespace {
 thread_local std::string g_noctx_error;
 
+// Device buffers allocated during a call come from the call context's pool.
+struct PoolScope {
+  std::shared_ptr<DevPool> saved;
+  explicit PoolScope(Ctx *c) : saved(current_pool()) { current_pool() = c ? c->pool : nullptr; }
+  ~PoolScope() { current_pool() = saved; }
+};
+
 template <class F>
 int guarded(Ctx *c, F &&f) {
   try {
+    PoolScope scope(c);
     if (c) QLG_HIP(hipSetDevice(c->device));
     f();
     return QLG_OK;
@@ -63,6 +72,8 @@ int qlg_ctx_create(int device, void *stream, qlg_ctx *out) {
     h->c.partials.ensure(2 * kMaxPartials * sizeof(double));
     h->c.scalars.ensure(64 * sizeof(double));
     QLG_HIP(hipHostMalloc((void **)&h->c.host_scalars, 64 * sizeof(double), hipHostMallocDefault));
+    const char *pool = std::getenv("QLG_POOL");
+    if (!(pool && pool[0] == '0')) h->c.pool = std::make_shared<DevPool>(device);
   });
   if (rc != QLG_OK) {
     delete h;
@@ -89,6 +100,30 @@ const char *qlg_last_error(qlg_ctx ctx) { return ctx ? ctx->c.last_error.c_str()
 int qlg_ctx_synchronize(qlg_ctx ctx) {
   if (!ctx) return QLG_ERR_INVALID;
   return guarded(&ctx->c, [&] { QLG_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+int qlg_ctx_release_cached(qlg_ctx ctx) {
+  if (!ctx) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    if (!ctx->c.pool) return;
+    QLG_HIP(hipStreamSynchronize(ctx->c.stream));
+    std::lock_guard<std::mutex> lk(ctx->c.pool->mu);
+    ctx->c.pool->trim(0);
+  });
+}
+
+int qlg_ctx_memory_stats(qlg_ctx ctx, int64_t stats[4]) {
+  if (!ctx || !stats) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    for (int i = 0; i < 4; ++i) stats[i] = 0;
+    if (!ctx->c.pool) return;
+    DevPool &pl = *ctx->c.pool;
+    std::lock_guard<std::mutex> lk(pl.mu);
+    stats[0] = (int64_t)pl.cached;
+    stats[1] = pl.n_malloc;
+    stats[2] = pl.n_reuse;
+    stats[3] = pl.n_free;
+  });
 }
 
 int qlg_device_alloc(qlg_ctx ctx, int64_t bytes, void **ptr) {

@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -34,28 +36,67 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::qlg::Error(QLG_ERR_INVALID, msg);                         \
   } while (0)
 
-// Device allocation owned by the library (plan workspace).
+// Device memory of the library.  Allocations made inside a call on a context
+// come from that context's block cache (DevPool): freed blocks are kept and
+// handed to later requests of a similar size instead of going back to the
+// driver, so the plans a sweep builds and destroys at every site do not call
+// hipMalloc / hipFree (a device-wide synchronisation) each time.  Blocks are
+// reused in the order of the context's stream: a block returned while work on
+// that stream still reads it is only handed to work queued after it on the
+// same stream (other streams fork from it with an event).  QLG_POOL=0 turns the
+// cache off; QLG_POOL_MB caps the bytes it holds (default: min(16 GiB, 1/8 of
+// the device)).
+void *dev_malloc(size_t n);     // hipMalloc, throwing QLG_ERR_ALLOC on failure
+void dev_free(void *p);
+
+struct DevPool {
+  int device = 0;
+  size_t cap = 0, cached = 0;
+  std::multimap<size_t, void *> blocks;   // cached blocks by size
+  int64_t n_malloc = 0, n_reuse = 0, n_free = 0;
+  std::mutex mu;
+  explicit DevPool(int dev);
+  ~DevPool();
+  void *get(size_t n, size_t *got);       // a block of at least n bytes
+  void put(void *p, size_t n);
+  void trim(size_t keep);                 // free cached blocks, largest first, down to keep bytes
+};
+
+// The pool that DevBuf::ensure draws from on this thread (set by the
+// entry-point guard for the call's context; null: plain hipMalloc).
+std::shared_ptr<DevPool> &current_pool();
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  std::shared_ptr<DevPool> pool;   // where p came from (null: hipMalloc)
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool)
+        pool->put(p, bytes);
+      else
+        dev_free(p);
+    }
     p = nullptr;
     bytes = 0;
+    pool.reset();
   }
   void ensure(size_t n) {
     if (n <= bytes && p) return;
     release();
     if (n == 0) n = 256;
-    if (hipMalloc(&p, n) != hipSuccess) {
-      p = nullptr;
-      throw Error(QLG_ERR_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
+    const std::shared_ptr<DevPool> &cp = current_pool();
+    if (cp) {
+      p = cp->get(n, &bytes);
+      pool = cp;
+    } else {
+      p = dev_malloc(n);
+      bytes = n;
     }
-    bytes = n;
   }
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
@@ -72,6 +113,7 @@ struct Ctx {
   DevBuf scalars;    // device-side scalar results
   double *host_scalars = nullptr;  // pinned, 64 doubles
   EighPool *eig = nullptr;         // lanes of the batched eigensolver, created on first use
+  std::shared_ptr<DevPool> pool;   // block cache of the plans' device buffers (null: off)
 };
 
 // ---------------------------------------------------------------------------

@@ -11,7 +11,8 @@
 // straight-line code, which lets the LDS fragment reads of one 4-deep
 // sub-step overlap the MFMAs of the previous one (a segment's last slice is
 // zero-filled past K and runs the same code).  Each wave owns a
-// (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks.
+// (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks; thin tiles (BN = 16, for
+// tasks with few columns) stack the four waves along M.
 //
 // MFMA f64 16x16x4 operand layout (one double per lane):
 //   A: lane l holds A[row = l % 16][k = l / 16]
@@ -37,11 +38,14 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
                                                                const GemmTask *__restrict__ tasks,
                                                                const GemmSeg *__restrict__ segs,
                                                                Slots slots, int ntiles) {
-  constexpr int WM = BM / 2, WN = BN / 2;       // wave sub-tile
+  // four waves: 2 x 2, or 4 x 1 for thin tiles (BN = 16)
+  constexpr int WAVES_N = BN >= 32 ? 2 : 1, WAVES_M = 4 / WAVES_N;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;   // wave sub-tile
   constexpr int FM = WM / 16, FN = WN / 16;     // MFMA blocks per wave
   constexpr int A_PER = BM * BK / 256;          // A elements loaded per thread per slice
   constexpr int B_PER = BK * BN / 256;
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small for 256 threads");
+  static_assert(WM >= 16 && WN >= 16, "wave sub-tile below one MFMA block");
   constexpr int NP = CPLX ? 2 : 1;              // real / imaginary planes
   constexpr uint32_t EB = CPLX ? 16u : 8u;      // element bytes
   __shared__ double As[2][NP][BK][BM + PAD];
@@ -49,7 +53,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
   if ((int)blockIdx.x >= ntiles) return;
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;  // padding slot of the XCD-balanced launch order
@@ -246,12 +250,16 @@ static void launch_shape(int bm, int bn, int bk, const GemmTile *tiles, const Ge
   QLG_SHAPE(64, 128, 16)
   QLG_SHAPE(128, 128, 16)
   if constexpr (!CPLX) {  // two accumulator planes of complex tiles do not fit BK=32 double buffers
+    QLG_SHAPE(64, 16, 16)
+    QLG_SHAPE(128, 16, 16)
     QLG_SHAPE(32, 32, 32)
     QLG_SHAPE(64, 32, 32)
     QLG_SHAPE(32, 64, 32)
     QLG_SHAPE(64, 64, 32)
     QLG_SHAPE(128, 64, 32)
     QLG_SHAPE(64, 128, 32)
+    QLG_SHAPE(64, 16, 32)
+    QLG_SHAPE(128, 16, 32)
   }
 #undef QLG_SHAPE
   throw Error(QLG_ERR_UNSUPPORTED, "unsupported FP64 tile shape");

@@ -841,8 +841,9 @@ bool autotune_enabled() {
 }
 }  // namespace
 
-// Tile autotuner of the two grouped-GEMM stages.  Every tile shape (seven
-// real, four complex; the cost model's ranking only orders them) is timed on
+// Tile autotuner of the two grouped-GEMM stages.  Every tile shape (nine for
+// real FP64, seven for the real reduced modes, four for complex; the cost
+// model's ranking only orders them) is timed on
 // this plan's own task lists and buffers (one warm-up and one timed launch each, HIP events
 // on the plan's stream); the fastest is kept.  Results are cached process-wide
 // under the stage's size-histogram signature, so sites with the same sector
@@ -881,7 +882,7 @@ void Plan::autotune(const double *in, double *out) {
         }
       }
       if (chosen.first == 0) {
-        const auto rank = rank_tile_shapes(g, cplx);
+        const auto rank = rank_tile_shapes(g, cplx, precision == QLG_PREC_F64 && !cplx);
         double best_ms = 1e300;
         for (size_t c = 0; c < rank.size(); ++c) {
           const auto sh = tile_shape(rank[c].second);

@@ -246,10 +246,14 @@ void validate_stage(const Stage &s, const int64_t *slot_elems) {
 // padded tiles at the FP64 peak, operand streaming at an L2-assisted rate,
 // the longest single tile, plus a fixed latency per tile.  Returns (estimate,
 // shape index) pairs, best first.
-static const int kShapes[][2] = {{32, 32}, {64, 32}, {32, 64}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
+// shapes 0-6 serve every kernel; the thin shapes 7-8 (four waves along M)
+// only the real FP64 one
+static const int kShapes[][2] = {{32, 32}, {64, 32}, {32, 64}, {64, 64}, {128, 64}, {64, 128}, {128, 128},
+                                 {64, 16}, {128, 16}};
 
-std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx) {
-  const int nshapes = cplx ? 4 : 7;   // complex tiles keep two accumulator planes: cap at 64x64
+std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx, bool thin) {
+  // complex tiles keep two accumulator planes: cap at 64x64
+  const int nshapes = cplx ? 4 : (thin ? 9 : 7);
   std::vector<int64_t> kpad(s.tasks.size(), 0);
   for (size_t t = 0; t < s.tasks.size(); ++t)
     for (int q = 0; q < s.tasks[t].seg_count; ++q) kpad[t] += (s.segs[s.tasks[t].seg_begin + q].K + 15) / 16 * 16;
@@ -278,7 +282,7 @@ std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx) 
 }
 
 void model_tiles(Stage &s, bool cplx) {
-  const auto r = rank_tile_shapes(s, cplx);
+  const auto r = rank_tile_shapes(s, cplx, false);
   s.make_tiles(kShapes[r[0].second][0], kShapes[r[0].second][1]);
 }
 

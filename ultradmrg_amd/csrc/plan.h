@@ -130,7 +130,7 @@ void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64,
 void launch_f64_to_f32(const double *src, float *dst, int64_t n, hipStream_t st);
 void launch_block_copy(bool cplx, const CopyDesc *d, int n, const double *src, double *dst, hipStream_t st);
 void validate_stage(const Stage &s, const int64_t *slot_elems);
-std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx);
+std::vector<std::pair<double, int>> rank_tile_shapes(const Stage &s, bool cplx, bool thin);
 void model_tiles(Stage &s, bool cplx);
 std::pair<int, int> tile_shape(int index);
 std::string tile_signature(const Stage &s, int stage, int variant);

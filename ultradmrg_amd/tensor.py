@@ -149,6 +149,16 @@ class Context:
     def synchronize(self):
         self.check(self.lib.qlg_ctx_synchronize(self.h))
 
+    def memory_stats(self) -> dict:
+        """The device block cache: cached bytes and its hipMalloc / reuse / hipFree counts."""
+        st = (C.c_int64 * 4)()
+        self.check(self.lib.qlg_ctx_memory_stats(self.h, st))
+        return {"cached_bytes": st[0], "mallocs": st[1], "reuses": st[2], "frees": st[3]}
+
+    def release_cached(self):
+        """Return the cached device blocks to the driver."""
+        self.check(self.lib.qlg_ctx_release_cached(self.h))
+
     def close(self):
         if self.h:
             self.lib.qlg_ctx_destroy(self.h)
