@@ -931,6 +931,39 @@ void Plan::autotune(const double *in, double *out) {
   if (verbose)
     std::fprintf(stderr, "[qlg tune] tiles %dx%d / %dx%d (cache hits %d, misses %d)\n", st[0].bm, st[0].bn, st[2].bm,
                  st[2].bn, g_tune_hits, g_tune_misses);
+  const char *vv = std::getenv("QLG_VERBOSE");
+  if (vv && vv[0] == '3') {
+    // task shapes of the two GEMM stages: FLOP-weighted quantiles of M, N, K
+    for (int i : {0, 2}) {
+      const Stage &g = st[i];
+      std::vector<std::array<double, 4>> v;   // flops, M, N, K
+      double tot = 0;
+      for (const GemmTask &k : g.tasks) {
+        int64_t K = 0;
+        for (int q = 0; q < k.seg_count; ++q) K += g.segs[k.seg_begin + q].K;
+        const double f = 2.0 * k.M * k.N * K;
+        v.push_back({f, double(k.M), double(k.N), double(K)});
+        tot += f;
+      }
+      std::string line;
+      for (int d = 1; d <= 3; ++d) {
+        std::sort(v.begin(), v.end(), [d](const auto &a, const auto &b) { return a[d] < b[d]; });
+        double acc = 0;
+        int qi = 0;
+        const double qs[3] = {0.1, 0.5, 0.9};
+        line += std::string(" ") + "MNK"[d - 1] + ":";
+        for (const auto &e : v) {
+          acc += e[0];
+          while (qi < 3 && acc >= qs[qi] * tot) {
+            line += " " + std::to_string((int64_t)e[d]);
+            ++qi;
+          }
+        }
+      }
+      std::fprintf(stderr, "[qlg shapes] stage %d: %zu tasks, %zu tiles of %dx%d, %.3f GFLOP;%s (FLOP-weighted 10/50/90%%)\n",
+                   i + 1, g.tasks.size(), g.tiles.size(), g.bm, g.bn, tot * 1e-9, line.c_str());
+    }
+  }
 }
 
 void Plan::matvec_fused(const double *in, double *out) {

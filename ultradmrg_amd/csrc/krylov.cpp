@@ -30,7 +30,11 @@ struct Workspace {
   int64_t n, nd;
   int64_t nslots;
   double *slot0;
-  double *dev_a, *dev_b, *dev_tmp;  // device scalars
+  // device scalars: a_m and b_m side by side (one copy fetches both), then
+  // scratch for norms and dots
+  double *ab, *dev_tmp;
+  double *A(int64_t m) { return ab + 2 * m; }
+  double *B(int64_t m) { return ab + 2 * m + 1; }
   Workspace(Plan &pl, int64_t maxit) : p(pl) {
     n = p.vec.size;
     nd = n * elem_doubles(p.dtype);
@@ -42,9 +46,8 @@ struct Workspace {
     }
     slot0 = p.basis.as<double>();
     p.scal_buf.ensure((size_t)(2 * (maxit + 2) + 8) * sizeof(double));
-    dev_a = p.scal_buf.as<double>();
-    dev_b = dev_a + (maxit + 2);
-    dev_tmp = dev_b + (maxit + 2);
+    ab = p.scal_buf.as<double>();
+    dev_tmp = ab + 2 * (maxit + 2);
     p.coef_buf.ensure((size_t)(2 * (maxit + 2)) * sizeof(double));
     p.ptr_buf.ensure((size_t)(maxit + 2) * sizeof(double *));
   }
@@ -111,11 +114,9 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
   vec_scal(c, p.dtype, w.n, 1.0 / h[0], 0.0, w.slot(0));
   // a_0 and the first residual
   p.matvec_fused(w.slot(0), w.slot(1));
-  vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0);
-  vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0, nullptr, nullptr, w.dev_b + 0);
-  QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  w.fetch(w.dev_tmp, h, 2);
+  vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.A(0));
+  vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.A(0), nullptr, nullptr, w.B(0));
+  w.fetch(w.A(0), h, 2);
   a[0] = h[0];
   double norm_gamma = h[1];
   double energy0 = a[0];
@@ -150,13 +151,10 @@ void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos
     b[m - 1] = norm_gamma;
     vec_scal(c, p.dtype, w.n, 1.0 / norm_gamma, 0.0, w.slot(m));
     p.matvec_fused(w.slot(m), w.slot(m + 1));
-    vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m);
+    vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.A(m));
     // speculative residual for iteration m+1: H v_m - a_m v_m - b_{m-1} v_{m-1}
-    vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m, w.slot(m - 1), w.dev_b + (m - 1),
-                         w.dev_b + m);
-    QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    w.fetch(w.dev_tmp, h, 2);
+    vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.A(m), w.slot(m - 1), w.B(m - 1), w.B(m));
+    w.fetch(w.A(m), h, 2);
     a[m] = h[0];
     norm_gamma = h[1];
     double e_new;
@@ -392,13 +390,12 @@ void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &p
   QLG_CHECK(initial_norm > 0.0, "initial state has zero norm");
   vec_scal(c, p.dtype, w.n, 1.0 / initial_norm, 0.0, w.slot(0));
   p.matvec_fused(w.slot(0), w.slot(1));
-  // a_m = Re <H v_m | v_m>; the complex dot writes (re, im)
-  vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_tmp + 2);
-  QLG_HIP(hipMemcpyAsync(w.dev_a + 0, w.dev_tmp + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.dev_a + 0, nullptr, nullptr, w.dev_b + 0);
-  QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  w.fetch(w.dev_tmp, h, 2);
+  // a_m = Re <H v_m | v_m>: the complex dot writes (re, im) over (a_m, b_m);
+  // the residual reads the real part and then overwrites the imaginary one
+  // with b_m
+  vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), w.A(0));
+  vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), w.A(0), nullptr, nullptr, w.B(0));
+  w.fetch(w.A(0), h, 2);
   a[0] = h[0];
   double norm_gamma = h[1];
   int64_t m = 0;
@@ -423,13 +420,9 @@ void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &p
     b[m - 1] = norm_gamma;
     vec_scal(c, p.dtype, w.n, 1.0 / norm_gamma, 0.0, w.slot(m));
     p.matvec_fused(w.slot(m), w.slot(m + 1));
-    vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_tmp + 2);
-    QLG_HIP(hipMemcpyAsync(w.dev_a + m, w.dev_tmp + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.dev_a + m, w.slot(m - 1), w.dev_b + (m - 1),
-                         w.dev_b + m);
-    QLG_HIP(hipMemcpyAsync(w.dev_tmp, w.dev_a + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    QLG_HIP(hipMemcpyAsync(w.dev_tmp + 1, w.dev_b + m, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    w.fetch(w.dev_tmp, h, 2);
+    vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.A(m));
+    vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), w.A(m), w.slot(m - 1), w.B(m - 1), w.B(m));
+    w.fetch(w.A(m), h, 2);
     a[m] = h[0];
     norm_gamma = h[1];
     std::fill(coef.begin(), coef.end(), 0.0);
