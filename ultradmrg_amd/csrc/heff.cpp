@@ -13,6 +13,7 @@
 // The Krylov vectors inside the library use the fused layout; the boundary
 // keeps the canonical one (pack / unpack copies at the solver's entry / exit).
 #include <algorithm>
+#include <iterator>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -113,11 +114,17 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
 
   pt.mark("index");
   // ---- symbolic reference schedule: closure of the block set + FLOPs --------
-  auto symbolic = [&](const std::set<Coords> &in, double *flops) {
-    std::set<Coords> out;
+  // Intermediate block sets are built as vectors, then sorted and
+  // de-duplicated once per stage (the lexicographic order of std::array).
+  auto sort_unique = [](auto &v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  auto symbolic = [&](const std::vector<Coords> &in, double *flops) {
+    std::vector<Coords> out;
     double f = 0, f1 = 0, f4 = 0;
     // stage T1: (a', w, s..., b)
-    std::set<Coords> t1;
+    std::vector<Coords> t1;
     for (const Coords &c : in) {
       auto it = L_by_ket.find(c[0]);
       if (it == L_by_ket.end()) continue;
@@ -129,15 +136,17 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         t[0] = l[0];
         t[1] = l[1];
         for (int i = 1; i < rk; ++i) t[1 + i] = c[i];
-        t1.insert(t);
+        t1.push_back(t);
         f1 += fl * double(L.ax[0].deg[l[0]] * L.ax[1].deg[l[1]]) * double(L.ax[2].deg[l[2]]) * double(nfree);
       }
     }
     // MPO stages: carry (a', [s' done...], w_cur, [s remaining...], b)
     // represent as Coords: [a', t_0..t_{k-1}, w, s_k..s_{n-1}, b]
-    std::set<Coords> cur = t1;  // k = 0: [a', w, s0, (s1), b]
+    sort_unique(t1);
+    std::vector<Coords> cur;  // k = 0: [a', w, s0, (s1), b]
+    cur.swap(t1);
     for (int k = 0; k < nphys; ++k) {
-      std::set<Coords> nxt;
+      std::vector<Coords> nxt;
       for (const Coords &c : cur) {
         const int w = c[1 + k], s = c[2 + k];
         auto it = W_by_in[k].find({w, s});
@@ -152,10 +161,11 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
           Coords t = c;
           t[1 + k] = wc[2];   // s_out
           t[2 + k] = wc[3];   // w right
-          nxt.insert(t);
+          nxt.push_back(t);
           f += fl * double(mfree) * double(kdim) * double(p.W[k].ax[2].deg[wc[2]] * p.W[k].ax[3].deg[wc[3]]);
         }
       }
+      sort_unique(nxt);
       cur.swap(nxt);
     }
     // cur: [a', t..., z, b] ; contract (z, b) with R
@@ -171,37 +181,41 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         o[0] = c[0];
         for (int j = 0; j < nphys; ++j) o[1 + j] = c[1 + j];
         o[bax] = R.blk[rb][0];
-        out.insert(o);
+        out.push_back(o);
         f4 += fl * double(m) * double(kd) * double(R.ax[0].deg[R.blk[rb][0]]);
       }
     }
-    if (flops) {  // [S1, S2 (+S3), S4] of the reference schedule
-      flops[0] = f1;
-      flops[1] = f;
-      flops[2] = f4;
-    }
+    sort_unique(out);
+    // [S1, S2 (+S3), S4] of the reference schedule
+    flops[0] = f1;
+    flops[1] = f;
+    flops[2] = f4;
     return out;
   };
 
-  std::set<Coords> vset(psi.blk.begin(), psi.blk.end());
-  for (int iter = 0; iter < 16; ++iter) {
-    std::set<Coords> out = symbolic(vset, nullptr);
-    size_t before = vset.size();
-    vset.insert(out.begin(), out.end());
-    if (vset.size() == before) break;
+  // closure of psi's blocks under H_eff; the FLOPs are those of the pass
+  // that finds the set closed
+  std::vector<Coords> vset(psi.blk.begin(), psi.blk.end());
+  sort_unique(vset);
+  double f3[3] = {0, 0, 0};
+  bool closed = false;
+  for (int iter = 0; iter < 16 && !closed; ++iter) {
+    const std::vector<Coords> out = symbolic(vset, f3);
+    std::vector<Coords> merged;
+    merged.reserve(vset.size() + out.size());
+    std::set_union(vset.begin(), vset.end(), out.begin(), out.end(), std::back_inserter(merged));
+    closed = merged.size() == vset.size();
+    vset.swap(merged);
   }
-  {
-    double f3[3];
-    symbolic(vset, f3);
-    p.algo_flops = f3[0] + f3[1] + f3[2];
-    for (int i = 0; i < 3; ++i) p.st[i].algo_flops = f3[i];
-  }
+  if (!closed) symbolic(vset, f3);
+  p.algo_flops = f3[0] + f3[1] + f3[2];
+  for (int i = 0; i < 3; ++i) p.st[i].algo_flops = f3[i];
 
   pt.mark("symbolic");
   // ---- vector layout ----------------------------------------------------------
   p.vec.rank = rk;
   p.vec.ax = psi.ax;
-  p.vec.blk.assign(vset.begin(), vset.end());   // std::set order = lexicographic
+  p.vec.blk.assign(vset.begin(), vset.end());   // sorted: lexicographic order
   p.vec.off.resize(p.vec.blk.size());
   {
     int64_t o = 0;
@@ -494,11 +508,16 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   std::vector<double> Ch;
   int64_t coff = 0;
   for (Chan &c : ch) {
-    // structurally reachable outputs that feed an existing R block and output block
-    std::set<Key7> outs;
-    for (const InRow &r : c.in) {
-      std::vector<std::pair<std::array<int, 2>, int>> fr{{{-1, -1}, r.wsec}};
-      const int ss[2] = {r.ssec0, r.ssec1};
+    // structurally reachable outputs that feed an existing R block and output
+    // block; they depend on an input row's sectors only, not its components
+    std::vector<std::array<int, 3>> trip;   // (w sector, s0 sector, s1 sector)
+    trip.reserve(c.in.size());
+    for (const InRow &r : c.in) trip.push_back({r.wsec, r.ssec0, r.ssec1});
+    sort_unique(trip);
+    std::vector<Key7> outs;
+    for (const auto &tr : trip) {
+      std::vector<std::pair<std::array<int, 2>, int>> fr{{{-1, -1}, tr[0]}};
+      const int ss[2] = {tr[1], tr[2]};
       for (int k = 0; k < nphys; ++k) {
         std::vector<std::pair<std::array<int, 2>, int>> nf;
         for (auto &f : fr) {
@@ -530,11 +549,12 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
           for (int a0 = 0; a0 < dt0; ++a0)
             for (int a1 = 0; a1 < dt1; ++a1)
               for (int zz = 0; zz < dz; ++zz)
-                outs.insert(Key7{f.first[0], a0, nphys == 2 ? f.first[1] : 0, a1, z, zz, bp});
+                outs.push_back(Key7{f.first[0], a0, nphys == 2 ? f.first[1] : 0, a1, z, zz, bp});
         }
       }
     }
-    c.out.assign(outs.begin(), outs.end());
+    sort_unique(outs);
+    c.out.swap(outs);
     if (c.out.empty() || c.in.empty()) continue;
     const int64_t Dap = L.ax[0].deg[c.ap], Db = psi.ax[bax].deg[c.b];
     ChanTask t{};
