@@ -23,14 +23,15 @@ sample of the same workload.
     python bench.py --copy-calibration   # 1 GiB device copy, for counter scaling
 
 Beside the FP64 matvec (``value``), the line carries the same matvec in the
-split-BF16 6-term mode (``split``) and, unless ``--no-sweep``, the sweep
+split-BF16 6-term mode (``split``) and the 3-term mode (``split_alt``) and,
+unless ``--no-sweep``, the sweep
 comparison of the headline metric (``sweep``): the 4x16 Hubbard cylinder
 (t=1, U=8, 56 electrons, Sz=0, the reference benchmark's initial product
 state, ``benchmark/ultradmrg_hubbard2d.cpp:130-248``) is ramped in FP64 through
 D = 10, 100, 500, 1000 (one sweep each) and three sweeps at D=2000, then two
 sweeps at D=2000 (Lanczos 1e-9 / 100, truncation 1e-8) are timed in FP64 and,
-from the same state, in the split-BF16 mode; energies are compared relative
-to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
+from the same state, in the split-BF16 6-term and 3-term modes; energies are
+compared relative to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
 random-field Heisenberg chains solved as one batch (one grouped launch per
 matvec stage, hipGraph-captured Lanczos iterations) with a loop over the
 chains; unless ``--no-tdvp``, ``tdvp`` times three complex128 TDVP steps at
@@ -81,9 +82,10 @@ def recorded_traffic(D, precision, stage):
     return float(v) if v is not None else None
 
 
-def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 2000, 2000, 2000), D=2000,
+def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 1000, 2000, 2000, 2000), D=2000,
                              nsweeps=2):
-    """Seconds per D=2000 sweep in FP64 and in the split mode, from one ramped state."""
+    """Seconds per D=2000 sweep in FP64, in the split mode and in the ``alt``
+    modes, each from the same ramped state."""
     import torch
     from ultradmrg_amd import mpo as M
     from ultradmrg_amd.solver import LanczosParams
@@ -110,7 +112,7 @@ def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 20
     saved = (list(drv.mps), list(drv.lenvs), list(drv.renvs))   # updates replace tensors, never mutate them
     out = {"model": f"Hubbard {lx}x4 cylinder, U=8, t=1, {n - n // per_hole} electrons, Sz=0",
            "ramp": list(ramp), "ramp_s": ramp_s, "ramp_energy": ramp_e, "D": D, "timed_sweeps": nsweeps}
-    for prec in ("fp64", split):
+    for prec in ("fp64", split) + tuple(alt):
         drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
         drv.p = SweepParams(1, D, D, 1e-8, lp, precision=prec, verbose=False)
         st = drv.stats
@@ -128,6 +130,8 @@ def hubbard_sweep_comparison(split="bf16x6", lx=16, ramp=(10, 100, 500, 1000, 20
     out["split"] = split
     out["speedup"] = sum(out["fp64"]["sweep_s"]) / sum(out[split]["sweep_s"])
     out["rel_dE"] = abs(es - e64) / abs(e64)
+    out["alt"] = {m: {"speedup": sum(out["fp64"]["sweep_s"]) / sum(out[m]["sweep_s"]),
+                      "rel_dE": abs(out[m]["energy"] - e64) / abs(e64)} for m in alt}
     return out
 
 
@@ -336,6 +340,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--split", default="bf16x6", help="reduced-precision mode compared with FP64")
+    ap.add_argument("--split-alt", default="bf16x3",
+                    help="second reduced-precision mode measured beside it in the matvec and sweep ('none': skip)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-chains", action="store_true")
     ap.add_argument("--no-tdvp", action="store_true")
@@ -408,8 +414,8 @@ def main():
         "roofline": roof,
     }
     # the same matvec with the split-precision sector GEMMs
-    if a.split != a.precision:
-        H.set_precision(a.split)
+    def split_line(mode):
+        H.set_precision(mode)
         ms_split = time_matvec(H, x, y, a.steps, a.warmup)
         H.enable_timing(True)
         acc = [0.0, 0.0, 0.0]
@@ -418,12 +424,18 @@ def main():
             acc = [p + q for p, q in zip(acc, H.stage_ms())]
         H.enable_timing(False)
         H.set_precision(a.precision)
-        mult = {"fp32": 1, "bf16x6": 6, "bf16x3": 3}.get(a.split, 1)
-        out["split"] = {"precision": a.split, "value": algo / (ms_split * 1e-3) / 1e12, "unit": "TFLOP/s",
-                        "ms_per_step": ms_split, "speedup_vs_fp64": ms / ms_split,
-                        "stage_ms": [v / nt for v in acc],
-                        "issued_mfma_tflops": execf * mult / (ms_split * 1e-3) / 1e12,
-                        "note": "algorithmic FLOPs per second; issued = executed FLOPs x products per term"}
+        mult = {"fp32": 1, "bf16x6": 6, "bf16x3": 3}.get(mode, 1)
+        return {"precision": mode, "value": algo / (ms_split * 1e-3) / 1e12, "unit": "TFLOP/s",
+                "ms_per_step": ms_split, "speedup_vs_fp64": ms / ms_split,
+                "stage_ms": [v / nt for v in acc],
+                "issued_mfma_tflops": execf * mult / (ms_split * 1e-3) / 1e12,
+                "note": "algorithmic FLOPs per second; issued = executed FLOPs x products per term"}
+
+    alt = () if a.split_alt in ("none", a.split, a.precision) else (a.split_alt,)
+    if a.split != a.precision:
+        out["split"] = split_line(a.split)
+    for mode in alt:
+        out["split_alt"] = split_line(mode)
     H.close()
     if not a.no_chain:
         out["heisenberg"] = heisenberg_ramp_measurement()
@@ -432,7 +444,7 @@ def main():
     if not a.no_tdvp:
         out["tdvp"] = tdvp_measurement(split=a.split)
     if not a.no_sweep:
-        out["sweep"] = hubbard_sweep_comparison(split=a.split)
+        out["sweep"] = hubbard_sweep_comparison(split=a.split, alt=alt)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(eff, psi)
     print(json.dumps(out), flush=True)
