@@ -120,10 +120,19 @@ class _SvdStructure:
             Cn = acc
             self.sect.append(dict(q=q, rows=g["rows"], cols=g["cols"], rks=rks, cks=cks, ro=ro, co=co, R=R, C=Cn,
                                   tr=R > Cn, n=min(R, Cn), m=max(R, Cn), blocks=g["blocks"]))
-        # buckets of sectors by padded short side; slot j of bucket b
-        self.buckets: Dict[int, List[int]] = {}
+        # buckets of sectors by padded short side; slot j of bucket b.  Real
+        # buckets are padded only to their largest member (a multiple of 16),
+        # not to the bucket's bound: the eigensolver's time grows with the
+        # order.  Complex buckets keep the bound, the order of the fixed
+        # random unitaries their solves are rotated by.
+        raw: Dict[int, List[int]] = {}
         for si, sc in enumerate(self.sect):
-            self.buckets.setdefault(_bucket(sc["n"]), []).append(si)
+            raw.setdefault(_bucket(sc["n"]), []).append(si)
+        self.buckets: Dict[int, List[int]] = {}
+        for b, sis in raw.items():
+            if b > 1 and not t.is_complex:
+                b = -(-max(self.sect[si]["n"] for si in sis) // 16) * 16
+            self.buckets[b] = sis
         self.mpad = {b: max(16, -(-max(self.sect[si]["m"] for si in sis) // 16) * 16)
                      for b, sis in self.buckets.items()}
         # per-block table: data offset, rows, cols, batch base, row/col origin in
