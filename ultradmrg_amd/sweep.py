@@ -120,17 +120,15 @@ class _SvdStructure:
             Cn = acc
             self.sect.append(dict(q=q, rows=g["rows"], cols=g["cols"], rks=rks, cks=cks, ro=ro, co=co, R=R, C=Cn,
                                   tr=R > Cn, n=min(R, Cn), m=max(R, Cn), blocks=g["blocks"]))
-        # buckets of sectors by padded short side; slot j of bucket b.  Real
-        # buckets are padded only to their largest member (a multiple of 16),
-        # not to the bucket's bound: the eigensolver's time grows with the
-        # order.  Complex buckets keep the bound, the order of the fixed
-        # random unitaries their solves are rotated by.
+        # buckets of sectors by padded short side; slot j of bucket b.  A
+        # bucket is padded only to its largest member (a multiple of 16), not
+        # to the bucket's bound: the eigensolver's time grows with the order.
         raw: Dict[int, List[int]] = {}
         for si, sc in enumerate(self.sect):
             raw.setdefault(_bucket(sc["n"]), []).append(si)
         self.buckets: Dict[int, List[int]] = {}
         for b, sis in raw.items():
-            if b > 1 and not t.is_complex:
+            if b > 1:
                 b = -(-max(self.sect[si]["n"] for si in sis) // 16) * 16
             self.buckets[b] = sis
         self.mpad = {b: max(16, -(-max(self.sect[si]["m"] for si in sis) // 16) * 16)
@@ -181,18 +179,43 @@ def _svd_structure(t: BlockTensor, ldims: int, ldiv) -> _SvdStructure:
     return st
 
 
-_UNITARY: Dict[tuple, torch.Tensor] = {}
+_REFLECTORS: Dict[tuple, torch.Tensor] = {}
 
 
-def _random_unitary(b: int, dtype, device, attempt: int = 0) -> torch.Tensor:
+def _reflectors(b: int, dtype, device, attempt: int = 0) -> torch.Tensor:
+    """Two fixed random unit vectors u_1, u_2 of length b ([2, b, 1]); the
+    Householder reflectors H_i = I - 2 u_i u_i^H are Hermitian and unitary, and
+    Q = H_2 H_1 turns a block-diagonal Hermitian matrix into a dense one."""
     key = (b, dtype, str(device), attempt)
-    q = _UNITARY.get(key)
-    if q is None:
+    u = _REFLECTORS.get(key)
+    if u is None:
+        if len(_REFLECTORS) > 512:
+            _REFLECTORS.clear()
         gen = torch.Generator(device="cpu").manual_seed(4321 + 7919 * attempt + b)
-        z = torch.complex(torch.randn(b, b, generator=gen, dtype=torch.float64),
-                          torch.randn(b, b, generator=gen, dtype=torch.float64))
-        q = _UNITARY[key] = torch.linalg.qr(z.to(device))[0].to(dtype)
-    return q
+        z = torch.complex(torch.randn(2, b, 1, generator=gen, dtype=torch.float64),
+                          torch.randn(2, b, 1, generator=gen, dtype=torch.float64))
+        z = z / torch.linalg.vector_norm(z, dim=1, keepdim=True)
+        u = _REFLECTORS[key] = z.to(device=device, dtype=dtype)
+    return u
+
+
+def _rotate_hermitian(G: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
+    """Q G Q^H for Q = H_2 H_1 and a batch G [B, n, n] of Hermitian matrices:
+    H G H = G - 2 u w^H - 2 w u^H + 4 (u^H w) u u^H with w = G u."""
+    for i in range(u.shape[0]):
+        ui = u[i]
+        w = G @ ui                                   # [B, n, 1]
+        sc = (ui.mH @ w).real                        # [B, 1, 1]
+        G = G - 2 * ui @ w.mH - 2 * w @ ui.mH + 4 * sc * (ui @ ui.mH)
+    return G
+
+
+def _unrotate_vectors(V: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
+    """Q^H V = H_1 H_2 V: eigenvectors of Q G Q^H back to those of G."""
+    for i in reversed(range(u.shape[0])):
+        ui = u[i]
+        V = V - 2 * ui @ (ui.mH @ V)
+    return V
 
 
 def eigh_batched(mats: List[torch.Tensor]) -> List[Tuple[torch.Tensor, torch.Tensor]]:
@@ -220,12 +243,12 @@ def eigh_batched(mats: List[torch.Tensor]) -> List[Tuple[torch.Tensor, torch.Ten
 def _eigh_grams(grams: Dict[int, torch.Tensor]) -> Dict[int, Tuple[torch.Tensor, torch.Tensor]]:
     """Eigen-decompositions of the padded Gram batches of a block SVD.
 
-    Complex batches are rotated by a fixed random unitary, Q G Q^H, before the
-    solve and the vectors rotated back: the complex divide-and-conquer solver
-    returns NaN vectors when the tridiagonal reduction meets exactly-zero
-    sub-columns, which block-diagonal (padded or reducible) Gram matrices
-    produce.  A batch that still comes back with NaN is solved again with
-    another rotation; a third failure raises."""
+    Complex batches are rotated by a fixed unitary Q = H_2 H_1 (two random
+    Householder reflectors), Q G Q^H, before the solve and the vectors rotated
+    back: the complex divide-and-conquer solver returns NaN vectors when the
+    tridiagonal reduction meets exactly-zero sub-columns, which block-diagonal
+    (padded or reducible) Gram matrices produce.  A batch that still comes back
+    with NaN is solved again with other reflectors; a third failure raises."""
     out, todo = {}, []
     for b, G in grams.items():
         if b == 1:
@@ -239,14 +262,14 @@ def _eigh_grams(grams: Dict[int, torch.Tensor]) -> Dict[int, Tuple[torch.Tensor,
             out[b] = r
         return out
     for attempt in range(3):
-        qs = {b: _random_unitary(b, grams[b].dtype, grams[b].device, attempt) for b in todo}
-        res = eigh_batched([(qs[b] @ grams[b] @ qs[b].mH).contiguous() for b in todo])
+        us = {b: _reflectors(b, grams[b].dtype, grams[b].device, attempt) for b in todo}
+        res = eigh_batched([_rotate_hermitian(grams[b], us[b]).contiguous() for b in todo])
         left = []
         for b, (w, V) in zip(todo, res):
             if bool(torch.isnan(V).any() or torch.isnan(w).any()):
                 left.append(b)
             else:
-                out[b] = (w, qs[b].mH @ V)
+                out[b] = (w, _unrotate_vectors(V, us[b]))
         todo = left
         if not todo:
             return out
