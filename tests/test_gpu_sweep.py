@@ -1,7 +1,8 @@
 """GPU parity of the contractions around the solve (tensordot, environment
 updates) against the oracle, and end-to-end two-site DMRG energies against the
-reference's known answers (tests/test_algorithm/test_finite_dmrg.cc:167-187,
-:729-791) and exact diagonalisation."""
+reference's known answers (tests/test_algorithm/test_finite_dmrg.cc:109-127,
+:167-254, :729-791; test_two_site_finite_tdvp.cc:102-123) and exact
+diagonalisation."""
 import numpy as np
 import pytest
 import torch
@@ -97,6 +98,65 @@ def test_dmrg_hubbard_2x2_kat():
     mps = direct_state_mps(m, [1 if i % 2 == 0 else 2 for i in range(4)])
     drv = TwoSiteVMPS(mps, M.build_mpo(m), SweepParams(10, 16, 16, 1e-9, LanczosParams(1e-8, 20), verbose=False))
     assert abs(drv.run() - (-2.828427124746)) < 1e-10
+
+
+def _spin_model(n, bonds, xy=True):
+    m = M.Model(n, M.SPIN_QNS, M.spin_operators())
+    for i, j in bonds:
+        m.add(1.0, (i, "sz"), (j, "sz"))
+        if xy:
+            m.add(0.5, (i, "sp"), (j, "sm"))
+            m.add(0.5, (i, "sm"), (j, "sp"))
+    return m
+
+
+def test_dmrg_ising_n6_kat():
+    """test_finite_dmrg.cc:109-127: H = sum Sz Sz on N=6, 4 sweeps, D 1..10, truncation 1e-5,
+    LanczosParams(1e-7): -0.25 (N - 1) to 1e-10.  The Neel start is an eigenstate, so every
+    Lanczos call ends at the first breakdown."""
+    print("[case] dmrg ising N=6", flush=True)
+    m = _spin_model(6, [(i, i + 1) for i in range(5)], xy=False)
+    mps = direct_state_mps(m, [i % 2 for i in range(6)])
+    drv = TwoSiteVMPS(mps, M.build_mpo(m), SweepParams(4, 1, 10, 1e-5, LanczosParams(1e-7), verbose=False))
+    assert abs(drv.run() - (-0.25 * 5)) < 1e-10
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_dmrg_heisenberg_2d_6site_kat(cplx):
+    """test_finite_dmrg.cc:207-254 (bonds :209-218), real and complex Hamiltonian: -3.129385241572 to 1e-12."""
+    print(f"[case] dmrg 2D heisenberg 6 sites complex={cplx}", flush=True)
+    dt = torch.complex128 if cplx else torch.float64
+    m = _spin_model(6, [(0, 1), (0, 2), (1, 3), (2, 3), (2, 4), (3, 5), (4, 5)])
+    mps = direct_state_mps(m, [i % 2 for i in range(6)], dtype=dt)
+    drv = TwoSiteVMPS(mps, M.build_mpo(m, dtype=dt), SweepParams(4, 8, 8, 1e-9, LanczosParams(1e-7), verbose=False))
+    assert abs(drv.run() - (-3.129385241572)) < 1e-12
+
+
+def test_dmrg_heisenberg_n6_complex_kat():
+    """test_finite_dmrg.cc:190-212: the N=6 chain with a complex Hamiltonian, -2.493577133888 to 1e-12."""
+    print("[case] dmrg heisenberg N=6 complex", flush=True)
+    m = M.heisenberg_chain(6)
+    mps = direct_state_mps(m, [i % 2 for i in range(6)], dtype=torch.complex128)
+    drv = TwoSiteVMPS(mps, M.build_mpo(m, dtype=torch.complex128),
+                      SweepParams(4, 8, 8, 1e-9, LanczosParams(1e-7), verbose=False))
+    assert abs(drv.run() - (-2.493577133888)) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_dmrg_free_fermion_n6_kat(cplx):
+    """test_two_site_finite_tdvp.cc:102-123: the spinless free-fermion chain N=6 (4 sweeps, D 1..16,
+    truncation 1e-10, LanczosParams(1e-8)), -3.4939592074349334893668128643185 to 1e-13; here as
+    three spin-up electrons of the U=0 Hubbard chain (N=3, 2Sz=3), the same Hamiltonian on that
+    sector."""
+    print(f"[case] dmrg free fermion N=6 complex={cplx}", flush=True)
+    dt = torch.complex128 if cplx else torch.float64
+    m = M.Model(6, M.HUBBARD_QNS, M.hubbard_operators())
+    for i in range(5):
+        M._hopping(m, 1.0, i, i + 1)
+    mps = direct_state_mps(m, [1, 3, 1, 3, 1, 3], dtype=dt)
+    drv = TwoSiteVMPS(mps, M.build_mpo(m, dtype=dt), SweepParams(4, 1, 16, 1e-10, LanczosParams(1e-8),
+                                                                   verbose=False))
+    assert abs(drv.run() - (-3.4939592074349334893668128643185)) < 1e-13
 
 
 def test_dmrg_config1_heisenberg_l20():

@@ -4,7 +4,38 @@ import pytest
 
 from oracle.dmrg import SweepParams, direct_state_mps, two_site_vmps, random_mps
 from oracle.lanczos import LanczosParams
-from oracle.models import build_mpo, ed_ground_energy, heisenberg_chain, hubbard_square, hubbard_cylinder
+from oracle.models import (HUBBARD_QNS, SPIN_QNS, ModelSpec, _add_hopping, build_mpo, ed_ground_energy, heisenberg_chain,
+                           hubbard_ops, hubbard_square, hubbard_cylinder, spin_ops)
+
+# test_finite_dmrg.cc:209-218: the 6-site 2D Heisenberg bonds
+HEIS_2D_BONDS = [(0, 1), (0, 2), (1, 3), (2, 3), (2, 4), (3, 5), (4, 5)]
+
+
+def ising_chain(n):
+    """test_finite_dmrg.cc:109-113: H = sum Sz_i Sz_{i+1}."""
+    m = ModelSpec(n, SPIN_QNS, spin_ops())
+    for i in range(n - 1):
+        m.add(1.0, (i, "sz"), (i + 1, "sz"))
+    return m
+
+
+def heisenberg_bonds(n, bonds):
+    m = ModelSpec(n, SPIN_QNS, spin_ops())
+    for i, j in bonds:
+        m.add(1.0, (i, "sz"), (j, "sz"))
+        m.add(0.5, (i, "sp"), (j, "sm"))
+        m.add(0.5, (i, "sm"), (j, "sp"))
+    return m
+
+
+def free_fermion_chain(n):
+    """Open chain H = -sum (c+_i c_{i+1} + h.c.) for both spins (Hubbard at U=0);
+    in the sector of spin-up electrons only it is the spinless free-fermion chain
+    of test_two_site_finite_tdvp.cc:102-106."""
+    m = ModelSpec(n, HUBBARD_QNS, hubbard_ops())
+    for i in range(n - 1):
+        _add_hopping(m, 1.0, i, i + 1)
+    return m
 
 
 def test_heisenberg_n6():
@@ -40,3 +71,34 @@ def test_random_mps_start():
     mps = random_mps(m, (0,), 16, seed=1001)
     e = two_site_vmps(mps, build_mpo(m), SweepParams(3, 32, 32, 1e-12, LanczosParams(1e-10, 100)))
     assert abs(e - ed_ground_energy(m, (0,))) < 1e-9
+
+
+def test_ising_n6():
+    """test_finite_dmrg.cc:109-127: -0.25 (N - 1) to 1e-10 (4 sweeps, D 1..10, LanczosParams(1e-7))."""
+    m = ising_chain(6)
+    mps = direct_state_mps(m, [i % 2 for i in range(6)])
+    e = two_site_vmps(mps, build_mpo(m), SweepParams(4, 1, 10, 1e-5, LanczosParams(1e-7)))
+    assert abs(e - (-0.25 * 5)) < 1e-10
+    assert abs(ed_ground_energy(m, (0,)) - (-0.25 * 5)) < 1e-12
+
+
+def test_heisenberg_2d_6site():
+    """test_finite_dmrg.cc:207-254: -3.129385241572 to 1e-12 (4 sweeps, D=8, LanczosParams(1e-7))."""
+    m = heisenberg_bonds(6, HEIS_2D_BONDS)
+    mps = direct_state_mps(m, [i % 2 for i in range(6)])
+    e = two_site_vmps(mps, build_mpo(m), SweepParams(4, 8, 8, 1e-9, LanczosParams(1e-7)))
+    assert abs(e - (-3.129385241572)) < 1e-12
+    assert abs(ed_ground_energy(m, (0,)) - (-3.129385241572)) < 1e-12
+
+
+def test_free_fermion_n6_energy():
+    """test_two_site_finite_tdvp.cc:121-123: -3.4939592074349334893668128643185 to 1e-13, here as three
+    spin-up electrons of the U=0 Hubbard chain (N=3, 2Sz=3); the closed form is
+    -2 (cos(pi/7) + cos(2 pi/7) + cos(3 pi/7))."""
+    ref = -3.4939592074349334893668128643185
+    assert abs(-2 * sum(np.cos(k * np.pi / 7) for k in (1, 2, 3)) - ref) < 1e-14
+    m = free_fermion_chain(6)
+    assert abs(ed_ground_energy(m, (3, 3)) - ref) < 1e-13
+    mps = direct_state_mps(m, [1, 3, 1, 3, 1, 3])
+    e = two_site_vmps(mps, build_mpo(m), SweepParams(4, 1, 16, 1e-10, LanczosParams(1e-8)))
+    assert abs(e - ref) < 1e-13
