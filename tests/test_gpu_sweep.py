@@ -159,6 +159,53 @@ def test_dmrg_free_fermion_n6_kat(cplx):
     assert abs(drv.run() - (-3.4939592074349334893668128643185)) < 1e-13
 
 
+def test_dmrg_kitaev_gamma_no_symmetry_kat():
+    """test_finite_dmrg.cc:346-445 (TestTwoSiteAlgorithmNoSymmetrySpinSystem.2DKitaevComplexCase):
+    J=-1, K=1, Gm=0.1, h=0.1 on a 3x4 lattice with no conserved charge and a complex Hamiltonian
+    (S^y terms); 4 sweeps at D=60, truncation 1e-4, LanczosParams(1e-10): -4.57509167674 to 2e-10.
+    That value is the exact ground-state energy (dense diagonalisation of the 4096 states gives
+    -4.575091676735409).  No symmetry: both spin states carry charge 0, so every tensor index
+    has two sectors of the same charge."""
+    print("[case] dmrg kitaev-gamma 3x4 no symmetry", flush=True)
+    J, K, Gm, h = -1.0, 1.0, 0.1, 0.1
+    nx, ny = 3, 4
+    n = nx * ny
+    sz = np.diag([0.5, -0.5]).astype(np.complex128)
+    sp_ = np.array([[0, 1], [0, 0]], dtype=np.complex128)
+    sx, sy = (sp_ + sp_.T) / 2, (sp_ - sp_.T) / 2j
+    # site operators are stored as op[s_in, s_out], the transpose of the matrix acting on kets
+    ops = {"sz": sz.T.copy(), "sx": sx.T.copy(), "sy": sy.T.copy(), "id": np.eye(2, dtype=np.complex128)}
+    m = M.Model(n, [(0,), (0,)], ops)
+    idx = lambda x, y: x * ny + y
+    link = {"x": [(J, "sz", "sz"), (J, "sx", "sx"), (J, "sy", "sy"), (K, "sx", "sx"), (Gm, "sz", "sy"), (Gm, "sy", "sz")],
+            "z": [(J, "sz", "sz"), (J, "sx", "sx"), (J, "sy", "sy"), (K, "sz", "sz"), (Gm, "sx", "sy"), (Gm, "sy", "sx")],
+            "y": [(J, "sz", "sz"), (J, "sx", "sx"), (J, "sy", "sy"), (K, "sy", "sy"), (Gm, "sz", "sx"), (Gm, "sx", "sz")]}
+    for x in range(nx):
+        for y in range(ny):
+            s0 = idx(x, y)
+            for o in ("sz", "sx", "sy"):
+                m.add(-h, (s0, o))
+            bonds = []
+            if y % 2 == 0:
+                bonds.append(("x", s0, idx(x, y + 1)))
+            if y % 2 == 1:
+                a, b = sorted((s0, idx(x, (y + 1) % ny)))
+                bonds.append(("z", a, b))
+            if y % 2 == 1 and x != nx - 1:
+                a, b = sorted((s0, idx(x + 1, y - 1)))
+                bonds.append(("y", a, b))
+            for kind, a, b in bonds:
+                for c, oa, ob in link[kind]:
+                    m.add(c, (a, oa), (b, ob))
+    labs = [i % 2 for i in range(n)]
+    dt = torch.complex128
+    mps = direct_state_mps(m, labs, dtype=dt)
+    drv = TwoSiteVMPS(mps, M.build_mpo(m, dtype=dt), SweepParams(4, 60, 60, 1e-4, LanczosParams(1e-10), verbose=False))
+    e = drv.run()
+    print(f"[kitaev-gamma] E0 {e:.12f}", flush=True)
+    assert abs(e - (-4.57509167674)) < 2e-10
+
+
 def test_dmrg_config1_heisenberg_l20():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
     truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""

@@ -139,6 +139,9 @@ def build_mpo(m: Model, device="cuda", dtype=torch.float64) -> List[BlockTensor]
     n, qns, ops = m.n, m.qns, m.ops
     d = len(qns)
     zero = tuple(0 for _ in qns[0])
+    cplx = any(np.iscomplexobj(o) and np.any(np.imag(o) != 0) for o in ops.values())
+    if cplx and not dtype.is_complex:
+        raise ValueError("the model has complex operators: build its MPO with a complex dtype")
     last: Dict[tuple, int] = {}
     for t in m.terms:
         if len(t.ops) == 2:
@@ -178,7 +181,7 @@ def build_mpo(m: Model, device="cuda", dtype=torch.float64) -> List[BlockTensor]
         rs = [("end",)] if k == n - 1 else bonds[k]
         la, lp = bond_axis(ls, IN)
         ra, rp = bond_axis(rs, OUT)
-        Wd = np.zeros((la.dim, d, d, ra.dim))
+        Wd = np.zeros((la.dim, d, d, ra.dim), dtype=np.complex128 if cplx else np.float64)
 
         def put(a, b, mat):
             if a in lp and b in rp:
