@@ -206,6 +206,23 @@ def test_dmrg_kitaev_gamma_no_symmetry_kat():
     assert abs(e - (-4.57509167674)) < 2e-10
 
 
+@pytest.mark.parametrize("case", ["1d", "2d"])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_dmrg_tj_kat(case, cplx):
+    """test_finite_dmrg.cc:484-583: t-J with t=3, J=1 on 4 sites, two electrons, Sz=0, real and complex
+    Hamiltonian; 1D chain -6.947478526233 (11 sweeps), 2D plaquette -8.868563739680 (10 sweeps), D=8,
+    LanczosParams(1e-8, 20), to 1e-10."""
+    from tests.workloads import TJ_1D_BONDS, TJ_2D_BONDS, tj_model
+    print(f"[case] dmrg t-J {case} complex={cplx}", flush=True)
+    bonds, labs, sweeps, ref = ((TJ_1D_BONDS, [2, 1, 2, 0], 11, -6.947478526233) if case == "1d" else
+                                (TJ_2D_BONDS, [2, 0, 1, 2], 10, -8.868563739680))
+    dt = torch.complex128 if cplx else torch.float64
+    m = tj_model(M.Model, 4, bonds, 3.0, 1.0)
+    drv = TwoSiteVMPS(direct_state_mps(m, labs, dtype=dt), M.build_mpo(m, dtype=dt),
+                      SweepParams(sweeps, 8, 8, 1e-9, LanczosParams(1e-8, 20), verbose=False))
+    assert abs(drv.run() - ref) < 1e-10
+
+
 def test_dmrg_config1_heisenberg_l20():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
     truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""

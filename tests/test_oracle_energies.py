@@ -102,3 +102,16 @@ def test_free_fermion_n6_energy():
     mps = direct_state_mps(m, [1, 3, 1, 3, 1, 3])
     e = two_site_vmps(mps, build_mpo(m), SweepParams(4, 1, 16, 1e-10, LanczosParams(1e-8)))
     assert abs(e - ref) < 1e-13
+
+
+@pytest.mark.parametrize("case", ["1d", "2d"])
+def test_tj(case):
+    """test_finite_dmrg.cc:484-583 (t=3, J=1, N=4, two electrons, Sz=0): 1D -6.947478526233 (11 sweeps),
+    2D plaquette -8.868563739680 (10 sweeps), D=8, LanczosParams(1e-8, 20), to 1e-10."""
+    from tests.workloads import TJ_1D_BONDS, TJ_2D_BONDS, tj_model
+    bonds, labs, sweeps, ref = ((TJ_1D_BONDS, [2, 1, 2, 0], 11, -6.947478526233) if case == "1d" else
+                                (TJ_2D_BONDS, [2, 0, 1, 2], 10, -8.868563739680))
+    m = tj_model(ModelSpec, 4, bonds, 3.0, 1.0)
+    assert abs(ed_ground_energy(m, (2, 0)) - ref) < 1e-10
+    e = two_site_vmps(direct_state_mps(m, labs), build_mpo(m), SweepParams(sweeps, 8, 8, 1e-9, LanczosParams(1e-8, 20)))
+    assert abs(e - ref) < 1e-10

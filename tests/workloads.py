@@ -62,3 +62,37 @@ def synthetic_eff_ham(kind, D, seed, nsect_half=None, width=None, site_model=Non
     for c in psi.allowed_blocks(zero):
         psi.blocks[c] = rng.uniform(-1, 1, size=psi.blk_shape(c))
     return [L, Wl, Wr, R], psi
+
+
+# t-J model of tests/test_algorithm/test_finite_dmrg.cc:446-583 (tJ_operators.h:110-128).
+# Site basis (up, down, empty) with U(1)xU(1) charges (N, 2Sz); operators stored op[s_in, s_out].
+TJ_QNS = [(1, 1), (1, -1), (0, 0)]
+
+
+def tj_operators():
+    def mk(entries):
+        o = np.zeros((3, 3))
+        for (i, j), v in entries.items():
+            o[i, j] = v
+        return o
+    return {"bupc": mk({(2, 0): 1}), "bupa": mk({(0, 2): 1}), "bdnc": mk({(2, 1): 1}), "bdna": mk({(1, 2): 1}),
+            "f": mk({(0, 0): -1, (1, 1): -1, (2, 2): 1}), "sz": mk({(0, 0): 0.5, (1, 1): -0.5}),
+            "sp": mk({(1, 0): 1}), "sm": mk({(0, 1): 1}), "id": np.eye(3)}
+
+
+def tj_model(model_cls, n, bonds, t, J):
+    """Hopping -t (c+_i c_j + h.c.) with the Jordan-Wigner string between the sites (no double
+    occupancy, so b+ F = b+ on the first site) and the Heisenberg coupling J S_i.S_j on each bond."""
+    m = model_cls(n, TJ_QNS, tj_operators())
+    for a, b in bonds:
+        a, b = min(a, b), max(a, b)
+        for o1, o2 in (("bupc", "bupa"), ("bdnc", "bdna"), ("bupa", "bupc"), ("bdna", "bdnc")):
+            m.add(-t, (a, o1), (b, o2), string="f")
+        m.add(J, (a, "sz"), (b, "sz"))
+        m.add(J / 2, (a, "sp"), (b, "sm"))
+        m.add(J / 2, (a, "sm"), (b, "sp"))
+    return m
+
+
+TJ_1D_BONDS = [(0, 1), (1, 2), (2, 3)]
+TJ_2D_BONDS = [(0, 1), (0, 2), (2, 3), (1, 3)]
