@@ -223,6 +223,25 @@ def test_dmrg_tj_kat(case, cplx):
     assert abs(drv.run() - ref) < 1e-10
 
 
+def test_dmrg_rashba_kat():
+    """test_finite_dmrg.cc:620-706: the Rashba t-J model, charge-only U(1) (sectors N=1 of degeneracy
+    two and N=0), complex Hamiltonian (imaginary spin-orbit coupling along y), 3x2 lattice periodic
+    in x (long-range terms with the Jordan-Wigner string), 8 sweeps at D=30, truncation 1e-4,
+    LanczosParams(1e-14, 100), start (up, down, up, empty, up, down): -11.018692166942165 to 1e-10."""
+    from tests.workloads import RASHBA_QNS, RASHBA_STD_OPS, rashba_terms
+    print("[case] dmrg rashba 3x2", flush=True)
+    ops = {k: v.T.copy() for k, v in RASHBA_STD_OPS.items()}     # stored op[s_in, s_out]
+    m = M.Model(6, RASHBA_QNS, ops)
+    for c, oa, a, ob, b, string in rashba_terms():
+        m.add(c, (a, oa), (b, ob), string=string)
+    dt = torch.complex128
+    drv = TwoSiteVMPS(direct_state_mps(m, [0, 1, 0, 2, 0, 1], dtype=dt), M.build_mpo(m, dtype=dt),
+                      SweepParams(8, 30, 30, 1e-4, LanczosParams(1e-14, 100), verbose=False))
+    e = drv.run()
+    print(f"[rashba] E0 {e:.15f}", flush=True)
+    assert abs(e - (-11.018692166942165)) < 1e-10
+
+
 def test_dmrg_config1_heisenberg_l20():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
     truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""

@@ -115,3 +115,27 @@ def test_tj(case):
     assert abs(ed_ground_energy(m, (2, 0)) - ref) < 1e-10
     e = two_site_vmps(direct_state_mps(m, labs), build_mpo(m), SweepParams(sweeps, 8, 8, 1e-9, LanczosParams(1e-8, 20)))
     assert abs(e - ref) < 1e-10
+
+
+def test_rashba_exact():
+    """test_finite_dmrg.cc:620-706: the Rashba t-J model (charge-only U(1), complex spin-orbit
+    coupling, 3x2 lattice periodic in x) has E0 = -11.018692166942165 at N=5: dense exact
+    diagonalisation of the model as transcribed for the GPU test."""
+    import itertools
+    from tests.workloads import RASHBA_STD_OPS as O, rashba_terms
+    n = 6
+
+    def op_at(d):
+        m = np.array([[1.0 + 0j]])
+        for k in range(n):
+            m = np.kron(m, d.get(k, np.eye(3)))
+        return m
+    H = np.zeros((3 ** n, 3 ** n), complex)
+    for c, oa, a, ob, b, string in rashba_terms():
+        d = {k: O["f"] for k in range(a + 1, b)} if string else {}
+        d[a], d[b] = O[oa], O[ob]
+        H += c * op_at(d)
+    assert np.abs(H - H.conj().T).max() < 1e-14
+    N = np.array([sum(1 for s in cfg if s < 2) for cfg in itertools.product(range(3), repeat=n)])
+    keep = np.where(N == 5)[0]
+    assert abs(np.linalg.eigvalsh(H[np.ix_(keep, keep)])[0] - (-11.018692166942165)) < 1e-12

@@ -96,3 +96,48 @@ def tj_model(model_cls, n, bonds, t, J):
 
 TJ_1D_BONDS = [(0, 1), (1, 2), (2, 3)]
 TJ_2D_BONDS = [(0, 1), (0, 2), (2, 3), (1, 3)]
+
+
+# Rashba t-J model of tests/test_algorithm/test_finite_dmrg.cc:598-706: charge-only U(1) (N), site basis
+# (up, down, empty), t=3, J=1, spin-orbit lambda=0.03 (imaginary along y), 3x2 lattice periodic in x.
+RASHBA_QNS = [(1,), (1,), (0,)]
+
+
+def rashba_terms(t=3.0, J=1.0, lamb=0.03, Nx=3, Ny=2):
+    """(coef, op_a, site_a, op_b, site_b, string) terms of test_finite_dmrg.cc:620-706."""
+    il = 1j * lamb
+    idx = lambda x, y: x * Ny + y
+    T = []
+    for x in range(Nx):
+        for y in range(Ny):
+            for kind in ("x", "y"):
+                if kind == "x":
+                    s0, s1 = sorted((idx(x, y), idx((x + 1) % Nx, y)))      # BCx periodic
+                    so = ([(lamb, "cdagup", "cdn"), (lamb, "cup", "cdagdn"), (-lamb, "cdagdn", "cup"), (-lamb, "cdn", "cdagup")]
+                          if x != Nx - 1 else
+                          [(lamb, "cdn", "cdagup"), (lamb, "cdagdn", "cup"), (-lamb, "cup", "cdagdn"), (-lamb, "cdagup", "cdn")])
+                else:
+                    if y == Ny - 1:                                          # BCy open
+                        continue
+                    s0, s1 = sorted((idx(x, y), idx(x, (y + 1) % Ny)))
+                    so = ([(il, "cdagup", "cdn"), (il, "cdagdn", "cup"), (-il, "cup", "cdagdn"), (-il, "cdn", "cdagup")]
+                          if y != Ny - 1 else
+                          [(il, "cup", "cdagdn"), (il, "cdn", "cdagup"), (-il, "cdagup", "cdn"), (-il, "cdagdn", "cup")])
+                for c, oa, ob in [(-t, "cdagup", "cup"), (-t, "cdagdn", "cdn"), (-t, "cup", "cdagup"), (-t, "cdn", "cdagdn")] + so:
+                    T.append((c, oa, s0, ob, s1, "f"))
+                for c, oa, ob in [(J, "sz", "sz"), (J / 2, "sp", "sm"), (J / 2, "sm", "sp"), (-J / 4, "ntot", "ntot")]:
+                    T.append((c, oa, s0, ob, s1, None))
+    return T
+
+def _std(entries):
+    """Matrix <out|O|in> over (up, down, empty)."""
+    o = np.zeros((3, 3), complex)
+    for (out, inn), v in entries.items():
+        o[out, inn] = v
+    return o
+
+
+RASHBA_STD_OPS = {"cup": _std({(2, 0): 1}), "cdagup": _std({(0, 2): 1}), "cdn": _std({(2, 1): 1}),
+                  "cdagdn": _std({(1, 2): 1}), "f": _std({(0, 0): -1, (1, 1): -1, (2, 2): 1}),
+                  "sz": _std({(0, 0): .5, (1, 1): -.5}), "sp": _std({(0, 1): 1}), "sm": _std({(1, 0): 1}),
+                  "ntot": _std({(0, 0): 1, (1, 1): 1})}

@@ -61,7 +61,7 @@ def hubbard_operators() -> Dict[str, np.ndarray]:
 
 @dataclass
 class Term:
-    coef: float
+    coef: float           # complex for complex couplings
     ops: List[Tuple[int, str]]
     string: Optional[str] = None
 
@@ -75,7 +75,8 @@ class Model:
 
     def add(self, coef, *site_ops, string=None):
         so = sorted(site_ops, key=lambda x: x[0])
-        self.terms.append(Term(float(coef), [(int(s), o) for s, o in so], string))
+        c = complex(coef)
+        self.terms.append(Term(c if c.imag != 0 else c.real, [(int(s), o) for s, o in so], string))
 
 
 def heisenberg_chain(n: int, J: float = 1.0, fields: Optional[Sequence[float]] = None) -> Model:
@@ -139,9 +140,10 @@ def build_mpo(m: Model, device="cuda", dtype=torch.float64) -> List[BlockTensor]
     n, qns, ops = m.n, m.qns, m.ops
     d = len(qns)
     zero = tuple(0 for _ in qns[0])
-    cplx = any(np.iscomplexobj(o) and np.any(np.imag(o) != 0) for o in ops.values())
+    cplx = (any(np.iscomplexobj(o) and np.any(np.imag(o) != 0) for o in ops.values()) or
+            any(isinstance(t.coef, complex) for t in m.terms))
     if cplx and not dtype.is_complex:
-        raise ValueError("the model has complex operators: build its MPO with a complex dtype")
+        raise ValueError("the model has complex operators or couplings: build its MPO with a complex dtype")
     last: Dict[tuple, int] = {}
     for t in m.terms:
         if len(t.ops) == 2:
