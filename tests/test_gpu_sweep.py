@@ -242,6 +242,25 @@ def test_dmrg_rashba_kat():
     assert abs(e - (-11.018692166942165)) < 1e-10
 
 
+def test_dmrg_kondo_mixed_sites_kat():
+    """test_finite_dmrg.cc:846-952: the Kondo double chain with sites of two kinds (4-state extended
+    electrons on even sites, 2-state localized spins on odd sites), no conserved charge, 5 sweeps at
+    D=64, LanczosParams(1e-8, 20), start i % 2: -3.180025784229132 to 1e-10."""
+    from tests.workloads import KONDO_STD_OPS, kondo_terms
+    print("[case] dmrg kondo double chain (mixed sites)", flush=True)
+    n = 8
+    ops = {k: v.T.copy() for k, v in KONDO_STD_OPS.items()}       # stored op[s_in, s_out]
+    qe, ql = [(0,)] * 4, [(0,)] * 2
+    m = M.Model(n, qe, ops, site_qns=[qe if i % 2 == 0 else ql for i in range(n)])
+    for c, oa, a, ob, b in kondo_terms():
+        m.add(c, (a, oa), (b, ob))
+    drv = TwoSiteVMPS(direct_state_mps(m, [i % 2 for i in range(n)]), M.build_mpo(m),
+                      SweepParams(5, 64, 64, 1e-9, LanczosParams(1e-8, 20), verbose=False))
+    e = drv.run()
+    print(f"[kondo] E0 {e:.15f}", flush=True)
+    assert abs(e - (-3.180025784229132)) < 1e-10
+
+
 def test_dmrg_config1_heisenberg_l20():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
     truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""

@@ -139,3 +139,21 @@ def test_rashba_exact():
     N = np.array([sum(1 for s in cfg if s < 2) for cfg in itertools.product(range(3), repeat=n)])
     keep = np.where(N == 5)[0]
     assert abs(np.linalg.eigvalsh(H[np.ix_(keep, keep)])[0] - (-11.018692166942165)) < 1e-12
+
+
+def test_kondo_exact():
+    """test_finite_dmrg.cc:846-952: the Kondo double chain (extended electrons on even sites, localized
+    spins on odd sites, t=0.25, Jk=1, Jz=0.5, 8 sites) has E0 = -3.180025784229132: dense exact
+    diagonalisation of the model as transcribed for the GPU test (4096 states)."""
+    from tests.workloads import KONDO_STD_OPS as O, kondo_terms
+    n = 8
+    dims = [4 if i % 2 == 0 else 2 for i in range(n)]
+
+    def op_at(d):
+        m = np.array([[1.0]])
+        for k in range(n):
+            m = np.kron(m, d.get(k, np.eye(dims[k])))
+        return m
+    H = sum(c * op_at({a: O[oa], b: O[ob]}) for c, oa, a, ob, b in kondo_terms())
+    assert np.abs(H - H.T).max() == 0.0
+    assert abs(np.linalg.eigvalsh(H)[0] - (-3.180025784229132)) < 1e-12

@@ -141,3 +141,33 @@ RASHBA_STD_OPS = {"cup": _std({(2, 0): 1}), "cdagup": _std({(0, 2): 1}), "cdn": 
                   "cdagdn": _std({(1, 2): 1}), "f": _std({(0, 0): -1, (1, 1): -1, (2, 2): 1}),
                   "sz": _std({(0, 0): .5, (1, 1): -.5}), "sp": _std({(0, 1): 1}), "sm": _std({(1, 0): 1}),
                   "ntot": _std({(0, 0): 1, (1, 1): 1})}
+
+
+# Kondo double chain of tests/test_algorithm/test_finite_dmrg.cc:846-952: sites of two kinds, an
+# extended electron (basis up, down, double, empty) on even sites and a localized spin (up, down) on
+# odd sites; no conserved charge is used (every state has charge 0).  Operators as <out|O|in>.
+def _mat(d, entries):
+    o = np.zeros((d, d))
+    for (out, inn), v in entries.items():
+        o[out, inn] = v
+    return o
+
+
+KONDO_STD_OPS = {
+    "sz": _mat(4, {(0, 0): 0.5, (1, 1): -0.5}), "sp": _mat(4, {(0, 1): 1}), "sm": _mat(4, {(1, 0): 1}),
+    "bupcF": _mat(4, {(2, 1): -1, (0, 3): 1}), "bupaF": _mat(4, {(1, 2): 1, (3, 0): -1}),
+    "Fbdnc": _mat(4, {(2, 0): 1, (1, 3): -1}), "Fbdna": _mat(4, {(0, 2): -1, (3, 1): 1}),
+    "bupc": _mat(4, {(2, 1): 1, (0, 3): 1}), "bupa": _mat(4, {(1, 2): 1, (3, 0): 1}),
+    "bdnc": _mat(4, {(2, 0): 1, (1, 3): 1}), "bdna": _mat(4, {(0, 2): 1, (3, 1): 1}),
+    "Sz": _mat(2, {(0, 0): 0.5, (1, 1): -0.5}), "Sp": _mat(2, {(0, 1): 1}), "Sm": _mat(2, {(1, 0): 1})}
+
+
+def kondo_terms(nx=4, t=0.25, jk=1.0, jz=0.5):
+    """(coef, op_a, site_a, op_b, site_b) terms of test_finite_dmrg.cc:925-937 (no string operators)."""
+    n, T = 2 * nx, []
+    for i in range(0, n - 2, 2):
+        T += [(-t, "bupcF", i, "bupa", i + 2), (-t, "bdnc", i, "Fbdna", i + 2), (t, "bupaF", i, "bupc", i + 2),
+              (t, "bdna", i, "Fbdnc", i + 2), (jz, "Sz", i + 1, "Sz", i + 3)]
+    for i in range(0, n, 2):
+        T += [(jk, "sz", i, "Sz", i + 1), (jk / 2, "sp", i, "Sm", i + 1), (jk / 2, "sm", i, "Sp", i + 1)]
+    return T

@@ -69,9 +69,13 @@ class Term:
 @dataclass
 class Model:
     n: int
-    qns: list
+    qns: list                              # charges of the site basis states
     ops: Dict[str, np.ndarray]
     terms: List[Term] = field(default_factory=list)
+    site_qns: Optional[list] = None        # per-site charge lists, for sites of different kinds
+
+    def qns_at(self, k: int) -> list:
+        return self.site_qns[k] if self.site_qns is not None else self.qns
 
     def add(self, coef, *site_ops, string=None):
         so = sorted(site_ops, key=lambda x: x[0])
@@ -137,9 +141,8 @@ def physical_axis(qns, dir=OUT) -> Axis:
 
 def build_mpo(m: Model, device="cuda", dtype=torch.float64) -> List[BlockTensor]:
     """Block-sparse MPO tensors on the device, one per site."""
-    n, qns, ops = m.n, m.qns, m.ops
-    d = len(qns)
-    zero = tuple(0 for _ in qns[0])
+    n, ops = m.n, m.ops
+    zero = tuple(0 for _ in m.qns_at(0)[0])
     cplx = (any(np.iscomplexobj(o) and np.any(np.imag(o) != 0) for o in ops.values()) or
             any(isinstance(t.coef, complex) for t in m.terms))
     if cplx and not dtype.is_complex:
@@ -161,7 +164,7 @@ def build_mpo(m: Model, device="cuda", dtype=torch.float64) -> List[BlockTensor]
     def qn_of(s):
         if s[0] != "open":
             return zero
-        return tuple(-v for v in _op_charge(ops[s[2]], qns))
+        return tuple(-v for v in _op_charge(ops[s[2]], m.qns_at(s[1])))
 
     def bond_axis(states, dir):
         groups: Dict[tuple, list] = {}
@@ -175,10 +178,12 @@ def build_mpo(m: Model, device="cuda", dtype=torch.float64) -> List[BlockTensor]
                 off += 1
         return Axis.make(secs, dir), pos
 
-    pout = physical_axis(qns, OUT)
-    pin = pout.inverse()
     out = []
     for k in range(n):
+        qns = m.qns_at(k)
+        d = len(qns)
+        pout = physical_axis(qns, OUT)
+        pin = pout.inverse()
         ls = [("start",)] if k == 0 else bonds[k - 1]
         rs = [("end",)] if k == n - 1 else bonds[k]
         la, lp = bond_axis(ls, IN)

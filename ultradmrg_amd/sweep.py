@@ -484,20 +484,21 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
 # ---------------------------------------------------------------------------
 def direct_state_mps(m: Model, stat_labs: Sequence[int], device="cuda", dtype=torch.float64) -> List[BlockTensor]:
     """``finite_mps_init.h:213-262``: product state, site 0 carries the total charge."""
-    qns, n = m.qns, m.n
-    pout = physical_axis(qns, OUT)
-    zero = tuple(0 for _ in qns[0])
+    n = m.n
+    qat = m.qns_at
+    zero = tuple(0 for _ in qat(0)[0])
     total = zero
     for i in range(n):
-        total = tuple(a + b for a, b in zip(total, qns[stat_labs[i]]))
+        total = tuple(a + b for a, b in zip(total, qat(i)[stat_labs[i]]))
     mps = []
     lvb = Axis.make([(zero, 1)], IN)
     lq = zero
     for i in range(n):
+        pout = physical_axis(qat(i), OUT)
         if i == 0:
-            rq = tuple(a - b for a, b in zip(total, qns[stat_labs[0]]))
+            rq = tuple(a - b for a, b in zip(total, qat(0)[stat_labs[0]]))
         elif i < n - 1:
-            rq = tuple(a - b for a, b in zip(lq, qns[stat_labs[i]]))
+            rq = tuple(a - b for a, b in zip(lq, qat(i)[stat_labs[i]]))
         else:
             rq = zero
         rvb = Axis.make([(rq, 1)], OUT)
