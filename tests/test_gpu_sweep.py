@@ -261,6 +261,24 @@ def test_dmrg_kondo_mixed_sites_kat():
     assert abs(e - (-3.180025784229132)) < 1e-10
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_dmrg_kitaev_simple_kat(cplx):
+    """test_finite_dmrg.cc:283-344: Sz Sz on the four z-links of a 4x2 honeycomb strip (sites
+    Ny (x % Nx) + y % Ny: links (1, 2), (0, 3), (5, 6), (4, 7)), 4 sweeps at D=8, truncation 1e-4,
+    LanczosParams(1e-10), from a random start (the reference superposes two product states;
+    here a seeded random MPS at D=8 in the Sz=0 sector): -1.0 to 1e-12, real and complex."""
+    print(f"[case] dmrg kitaev simple 4x2 complex={cplx}", flush=True)
+    nx, ny = 4, 2
+    hc = lambda x, y: ny * (x % nx) + y % ny
+    bonds = [tuple(sorted((hc(x, y), hc(x - 1, y + 1)))) for x in range(nx) for y in range(ny) if x % 2 == 1]
+    assert sorted(bonds) == [(0, 3), (1, 2), (4, 7), (5, 6)]
+    dt = torch.complex128 if cplx else torch.float64
+    m = _spin_model(nx * ny, bonds, xy=False)
+    mps = random_mps(m, (0,), 8, seed=283, dtype=dt)
+    drv = TwoSiteVMPS(mps, M.build_mpo(m, dtype=dt), SweepParams(4, 8, 8, 1e-4, LanczosParams(1e-10), verbose=False))
+    assert abs(drv.run() - (-1.0)) < 1e-12
+
+
 def test_dmrg_config1_heisenberg_l20():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
     truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""
