@@ -349,6 +349,21 @@ def test_matvec_every_tile_shape(monkeypatch, shape, kind, D, nh, seed):
     assert rel_err(to_oracle(out, ref), ref) < 1e-12
 
 
+@pytest.mark.parametrize("shape", ["32x32", "64x32", "32x64", "64x64", "128x64", "64x128", "128x128"])
+@pytest.mark.parametrize("prec,tol", [("bf16x6", 2e-6), ("bf16x3", 2e-4)])
+def test_reduced_precision_every_tile_shape(monkeypatch, shape, prec, tol):
+    """Every tile shape of the split-BF16 sector GEMMs, pinned for both GEMM stages, against the
+    FP64 oracle at the split modes' bars (Hubbard, D=160)."""
+    print(f"[case] reduced matvec tile {shape} {prec}", flush=True)
+    monkeypatch.setenv("QLG_TILES", f"{shape},32x128,{shape}")
+    eff, psi = synthetic_eff_ham("hubbard", 160, seed=4, nsect_half=3)
+    ref = eff_ham_mul_two_site_state(eff, psi)
+    d_eff = [to_device(t) for t in eff]
+    H = EffectiveHamiltonian(d_eff, to_device(psi), precision=prec)
+    out = H @ to_device(psi)
+    assert rel_err(to_oracle(out, ref), ref) < tol
+
+
 def _complexify(t: BSTensor, rng) -> BSTensor:
     c = BSTensor(t.indexes, np.complex128, t.div())
     c.blocks = {k: v + 1j * rng.uniform(-1, 1, size=v.shape) for k, v in t.blocks.items()}

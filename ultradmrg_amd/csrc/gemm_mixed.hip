@@ -16,8 +16,9 @@
 //                ~16 significant bits per operand;
 //      NPC = 6:  h.h + h.m + m.h + h.l + l.h + m.m  with three pieces,
 //                ~24 bits (FP32-like).
-//    Operands are staged in LDS as FP32 and split when the MFMA fragments
-//    are read, so LDS holds one copy whatever the number of pieces.
+//    In the 6-term mode, stage 3's real operands are split once, when their K
+//    slice is staged, and LDS holds one bf16 plane per piece; otherwise
+//    operands are staged as FP32 and split when the MFMA fragments are read.
 //
 // MFMA layouts (measured on gfx950 with an in-bounds probe):
 //   f32 16x16x4:   A lane l: A[l % 16][l / 16];  B lane l: B[l / 16][l % 16]
@@ -200,8 +201,9 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const GemmTile *__restric
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16.  LDS: As[m][k], Bs[n][k] (FP32, k contiguous, 32-deep slices),
-// ping-pong; fragments are split into bf16 pieces in registers.
+// Split-BF16 slices are 32 deep.  The fragment-split real kernel and the
+// complex kernel stage FP32 slices (As[m][k], Bs[n][k], k contiguous) and
+// split fragments in registers.
 namespace {
 constexpr int BKB = 32;
 constexpr int KP = BKB + 4;   // row pitch (floats)
@@ -221,8 +223,14 @@ __device__ inline void split8(const float (&x)[8], bf16x8 (&p)[NPIECE]) {
 }
 }  // namespace
 
+// Split-BF16, real data, split at the fragment read: LDS holds the FP32
+// slices (As[m][k], Bs[n][k], two thirds of the bytes of three piece planes)
+// and each wave splits the fragments it reads.  Used for stage 1 (short K:
+// few slices per tile, where a second resident workgroup per CU hides the
+// tile's prologue and epilogue) and for the 3-term mode (two pieces: less
+// split work to save).
 template <int BM, int BN, int NPC, typename TB, typename TC>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmTile *__restrict__ tiles,
+__global__ __launch_bounds__(256) void gemm_bf16_frag_kernel(const GemmTile *__restrict__ tiles,
                                                         const GemmTask *__restrict__ tasks,
                                                         const GemmSeg *__restrict__ segs, Slots slots, int ntiles) {
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
@@ -318,6 +326,189 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmTile *__restri
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][1], c, 0, 0, 0);
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][0], c, 0, 0, 0);
         acc[i][j] = c;
+      }
+  };
+
+  const int s_end = tk.seg_begin + tk.seg_count;
+  int s = tk.seg_begin;
+  while (s < s_end && segs[s].K <= 0) ++s;
+  if (s < s_end) {
+    GemmSeg cur = segs[s];
+    int k0 = 0, buf = 0;
+    load(cur, 0);
+    store(0, cur.flags);
+    __syncthreads();
+    while (true) {
+      int ns = s, nk = k0 + BKB;
+      GemmSeg nxt = cur;
+      if (nk >= cur.K) {
+        ++ns;
+        nk = 0;
+        while (ns < s_end && segs[ns].K <= 0) ++ns;
+        if (ns < s_end) nxt = segs[ns];
+      }
+      const bool more = ns < s_end;
+      if (more) load(nxt, nk);
+      step(buf);
+      if (!more) break;
+      store(buf ^ 1, nxt.flags);
+      __syncthreads();
+      buf ^= 1;
+      s = ns;
+      k0 = nk;
+      cur = nxt;
+    }
+  }
+  store_tile<FM, FN, TC>(tk, tl, slots, wm0, wn0, lane, acc);
+}
+
+// Split-BF16, real data.  Each operand value is split into its bf16 pieces
+// once, when its K slice is staged: LDS holds one bf16 plane per piece,
+// Ap[buf][piece][m][.] and Bp[buf][piece][n][.] (32-deep slices: a row is
+// four 16-byte chunks of 8 consecutive k), and every MFMA fragment (8
+// consecutive k of one row) is one 16-byte read of a plane.  The chunks of a
+// row are stored XOR-swizzled by (row / 4) % 4, so the 16 rows one fragment
+// read touches land on distinct banks.  Threads stage K in quads of four
+// consecutive values: A quads run along a row (coalesced); B quads run along
+// n when B is stored [K x N] and along k when it is stored [N x K].
+namespace {
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline int swz(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
+
+template <int NPIECE>
+__device__ inline void split4(const float (&x)[4], bf16x4 (&p)[NPIECE]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float r = x[j];
+#pragma unroll
+    for (int q = 0; q < NPIECE; ++q) {
+      const __bf16 h = (__bf16)r;
+      p[q][j] = h;
+      r -= (float)h;
+    }
+  }
+}
+}  // namespace
+
+template <int BM, int BN, int NPC, typename TB, typename TC>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmTile *__restrict__ tiles,
+                                                        const GemmTask *__restrict__ tasks,
+                                                        const GemmSeg *__restrict__ segs, Slots slots, int ntiles) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int NPIECE = NPC == 3 ? 2 : 3;
+  constexpr int KQ = BKB / 4;                              // quads per row of a slice
+  constexpr int QA = BM * KQ / 256, QB = BN * KQ / 256;    // quads per thread
+  static_assert(QA >= 1 && QB >= 1 && BM * KQ % 256 == 0 && BN * KQ % 256 == 0, "tile too small for 256 threads");
+  __shared__ bf16x8 Ap[2][NPIECE][BM][BKB / 8];
+  __shared__ bf16x8 Bp[2][NPIECE][BN][BKB / 8];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  if ((int)blockIdx.x >= ntiles) return;
+  const GemmTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GemmTask tk = tasks[tl.task];
+  const int M = tk.M, N = tk.N;
+
+  flt4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+  float ra[QA][4], rb[QB][4];
+
+  // quad i of this thread: row / column and quad index within the slice
+  auto a_quad = [&](int i, int &m, int &kq) {
+    const int q = tid + i * 256;
+    m = q / KQ;
+    kq = q % KQ;
+  };
+  auto b_quad = [&](int i, bool tb, int &n, int &kq) {
+    const int q = tid + i * 256;
+    n = tb ? q / KQ : q % BN;
+    kq = tb ? q % KQ : q / BN;
+  };
+  auto load = [&](const GemmSeg &sg, int k0) {
+    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
+    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
+    const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
+    const bool tb = sg.flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < QA; ++i) {
+      int m, kq;
+      a_quad(i, m, kq);
+      const uint32_t gm = tl.m0 + m;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t gk = k0 + 4 * kq + j;
+        ra[i][j] = load_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < QB; ++i) {
+      int n, kq;
+      b_quad(i, tb, n, kq);
+      const uint32_t gn = tl.n0 + n;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t gk = k0 + 4 * kq + j;
+        rb[i][j] = load_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+      }
+    }
+  };
+  auto store = [&](int buf, uint8_t flags) {
+    const bool tb = flags & SEG_TRANS_B;
+#pragma unroll
+    for (int i = 0; i < QA; ++i) {
+      int m, kq;
+      a_quad(i, m, kq);
+      bf16x4 pc[NPIECE];
+      split4<NPIECE>(ra[i], pc);
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p)
+        reinterpret_cast<bf16x4 *>(&Ap[buf][p][m][swz(m, kq >> 1)])[kq & 1] = pc[p];
+    }
+#pragma unroll
+    for (int i = 0; i < QB; ++i) {
+      int n, kq;
+      b_quad(i, tb, n, kq);
+      bf16x4 pc[NPIECE];
+      split4<NPIECE>(rb[i], pc);
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p)
+        reinterpret_cast<bf16x4 *>(&Bp[buf][p][n][swz(n, kq >> 1)])[kq & 1] = pc[p];
+    }
+  };
+  auto step = [&](int buf) {
+    bf16x8 ap[FM][NPIECE], bp[FN][NPIECE];
+    const int c = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int row = wm0 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p) ap[i][p] = Ap[buf][p][row][swz(row, c)];
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int row = wn0 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p) bp[j][p] = Bp[buf][p][row][swz(row, c)];
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        flt4 cc = acc[i][j];
+        // small products first
+        if constexpr (NPC == 6) {
+          cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][1], bp[j][1], cc, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][2], bp[j][0], cc, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][2], cc, 0, 0, 0);
+        }
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][1], bp[j][0], cc, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][1], cc, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][0], cc, 0, 0, 0);
+        acc[i][j] = cc;
       }
   };
 
@@ -567,10 +758,18 @@ static void launch_mixed_typed(int mode, int bm, int bn, const GemmTile *tiles, 
                        ntiles);                                                                               \
     return;                                                                                                   \
   }
+// Split once at staging only for stage 3 (FP32 operands, long K) in the
+// 6-term mode, where the split work is largest; stage 1 (B = the FP64 Krylov
+// vector, short K) and the 3-term mode split at the fragment read (measured
+// faster at D=2000 for both the synthetic and a converged Hubbard state).
 #define QLG_BF(M_, N_, P_)                                                                                    \
   if (bm == M_ && bn == N_) {                                                                                 \
-    hipLaunchKernelGGL((gemm_bf16_kernel<M_, N_, P_, TB, TC>), grid, block, 0, st, tiles, tasks, segs, slots, \
-                       ntiles);                                                                               \
+    if constexpr (sizeof(TB) == 8 || P_ == 3)                                                                 \
+      hipLaunchKernelGGL((gemm_bf16_frag_kernel<M_, N_, P_, TB, TC>), grid, block, 0, st, tiles, tasks, segs, \
+                         slots, ntiles);                                                                      \
+    else                                                                                                      \
+      hipLaunchKernelGGL((gemm_bf16_kernel<M_, N_, P_, TB, TC>), grid, block, 0, st, tiles, tasks, segs,      \
+                         slots, ntiles);                                                                      \
     return;                                                                                                   \
   }
   if (mode == QLG_PREC_F32) {
