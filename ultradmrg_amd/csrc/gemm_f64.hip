@@ -84,19 +84,48 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
     n = tb ? e / BK : e % BN;
     k = tb ? e % BK : e / BN;
   };
-  auto load = [&](const GemmSeg &sg, int k0) {
-    const __amdgpu_buffer_rsrc_t ar = make_rsrc(slots.p[sg.a_slot] + sg.a_off * NP);
-    const __amdgpu_buffer_rsrc_t br = make_rsrc(slots.p[sg.b_slot] + sg.b_off * NP);
+  // Per-segment state of the slices being loaded: buffer descriptors, and
+  // each element's offset at k0 = 0, its k within the slice and whether its
+  // row / column lies inside the task.  A slice then costs one add, one
+  // compare and one select per element.
+  __amdgpu_buffer_rsrc_t ar, br;
+  uint32_t astep = 1, bstep = 1;
+  int segK = 0;
+  uint32_t aoff[A_PER], boff[B_PER];
+  int akk[A_PER], bkk[B_PER];
+  bool ain[A_PER], bin[B_PER];
+  auto setup = [&](const GemmSeg &sg) {
+    ar = make_rsrc(slots.p[sg.a_slot] + sg.a_off * NP);
+    br = make_rsrc(slots.p[sg.b_slot] + sg.b_off * NP);
     const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
     const bool ta = sg.flags & SEG_TRANS_A, tb = sg.flags & SEG_TRANS_B;
-    const int K = sg.K;
+    segK = sg.K;
+    astep = ta ? lda : 1u;
+    bstep = tb ? 1u : ldb;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
       int m, k;
       a_mk(i, ta, m, k);
-      const uint32_t gm = tl.m0 + m, gk = k0 + k;
-      const uint32_t ix = ta ? gk * lda + gm : gm * lda + gk;
-      const uint32_t off = ((int)gm < M && (int)gk < K) ? ix * EB : kOob;
+      const uint32_t gm = tl.m0 + m;
+      aoff[i] = ta ? (uint32_t)k * lda + gm : gm * lda + (uint32_t)k;
+      akk[i] = k;
+      ain[i] = (int)gm < M;
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      int k, n;
+      b_kn(i, tb, k, n);
+      const uint32_t gn = tl.n0 + n;
+      boff[i] = tb ? gn * ldb + (uint32_t)k : (uint32_t)k * ldb + gn;
+      bkk[i] = k;
+      bin[i] = (int)gn < N;
+    }
+  };
+  auto load = [&](int k0) {
+    const uint32_t ka = (uint32_t)k0 * astep, kb = (uint32_t)k0 * bstep;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const uint32_t off = (ain[i] && akk[i] + k0 < segK) ? (aoff[i] + ka) * EB : kOob;
       if constexpr (CPLX) {
         const double2 v = load_c128(ar, off);
         ra[0][i] = v.x;
@@ -107,11 +136,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int k, n;
-      b_kn(i, tb, k, n);
-      const uint32_t gn = tl.n0 + n, gk = k0 + k;
-      const uint32_t ix = tb ? gn * ldb + gk : gk * ldb + gn;
-      const uint32_t off = ((int)gn < N && (int)gk < K) ? ix * EB : kOob;
+      const uint32_t off = (bin[i] && bkk[i] + k0 < segK) ? (boff[i] + kb) * EB : kOob;
       if constexpr (CPLX) {
         const double2 v = load_c128(br, off);
         rb[0][i] = v.x;
@@ -167,7 +192,8 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
   if (s < s_end) {
     GemmSeg cur = segs[s];
     int k0 = 0, buf = 0;
-    load(cur, 0);
+    setup(cur);
+    load(0);
     store(0, cur.flags);
     __syncthreads();
     while (true) {
@@ -181,7 +207,10 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
         if (ns < s_end) nxt = segs[ns];
       }
       const bool more = ns < s_end;
-      if (more) load(nxt, nk);
+      if (more) {
+        if (ns != s) setup(nxt);
+        load(nk);
+      }
       // the staged slice is zero-filled past the segment's K, so a partial
       // slice runs the same straight-line code as a full one
 #pragma unroll
