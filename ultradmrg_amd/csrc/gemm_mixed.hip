@@ -122,23 +122,45 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const GemmTile *__restric
     n = tb ? e / BK : e % BN;
     k = tb ? e % BK : e / BN;
   };
-  auto load = [&](const GemmSeg &sg, int k0) {
-    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
-    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
+  // Per-segment addressing (set up at each segment switch).  Since 256 is a
+  // multiple of the slice depth and of BN, element i of this thread sits at
+  // an affine position: offset off0 + i * soff (plus the slice's k0 term),
+  // k = kt + i * sk and row (column) r0 + i * sr, with the steps uniform.
+  __amdgpu_buffer_rsrc_t ar, br;
+  uint32_t a_off0 = 0, b_off0 = 0, a_soff = 0, b_soff = 0, bstep = 1;
+  int a_kt = 0, b_kt = 0, b_sk = 0, a_r0 = 0, b_r0 = 0, b_sr = 0, segK = 0;
+  auto setup = [&](const GemmSeg &sg) {
+    ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
+    br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
     const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
     const bool tb = sg.flags & SEG_TRANS_B;
+    segK = sg.K;
+    // A element e = tid + 256 i: (m, k) = (e / BK, e % BK)
+    a_r0 = tl.m0 + tid / BK;
+    a_kt = tid % BK;
+    a_off0 = (uint32_t)a_r0 * lda + (uint32_t)a_kt;
+    a_soff = (uint32_t)(256 / BK) * lda;
+    // B element e: (k, n) = (e / BN, e % BN), or (e % BK, e / BK) when transposed
+    const int bn = tb ? tid / BK : tid % BN, bk = tb ? tid % BK : tid / BN;
+    b_r0 = tl.n0 + bn;
+    b_kt = bk;
+    b_sr = tb ? 256 / BK : 0;
+    b_sk = tb ? 0 : 256 / BN;
+    b_off0 = tb ? (uint32_t)b_r0 * ldb + (uint32_t)bk : (uint32_t)bk * ldb + (uint32_t)b_r0;
+    b_soff = (uint32_t)(tb ? 256 / BK : 256 / BN) * ldb;
+    bstep = tb ? 1u : ldb;
+  };
+  auto load = [&](int k0) {
+    const uint32_t kb = (uint32_t)k0 * bstep;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256, m = e / BK, k = e % BK;
-      const uint32_t gm = tl.m0 + m, gk = k0 + k;
-      ra[i] = load_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+      const bool ok = a_r0 + i * (256 / BK) < M && a_kt + k0 < segK;
+      ra[i] = load_as_f32<float>(ar, ok ? a_off0 + (uint32_t)i * a_soff + (uint32_t)k0 : kOob);
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int k, n;
-      b_kn(i, tb, k, n);
-      const uint32_t gn = tl.n0 + n, gk = k0 + k;
-      rb[i] = load_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+      const bool ok = b_r0 + i * b_sr < N && b_kt + i * b_sk + k0 < segK;
+      rb[i] = load_as_f32<TB>(br, ok ? b_off0 + (uint32_t)i * b_soff + kb : kOob);
     }
   };
   auto store = [&](int buf, uint8_t flags) {
@@ -162,7 +184,8 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const GemmTile *__restric
   if (s < s_end) {
     GemmSeg cur = segs[s];
     int k0 = 0, buf = 0;
-    load(cur, 0);
+    setup(cur);
+    load(0);
     store(0, cur.flags);
     __syncthreads();
     while (true) {
@@ -175,7 +198,10 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const GemmTile *__restric
         if (ns < s_end) nxt = segs[ns];
       }
       const bool more = ns < s_end;
-      if (more) load(nxt, nk);
+      if (more) {
+        if (ns != s) setup(nxt);
+        load(nk);
+      }
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         float af[FM], bf[FN];
@@ -259,23 +285,45 @@ __global__ __launch_bounds__(256) void gemm_bf16_frag_kernel(const GemmTile *__r
     n = tb ? e / BKB : e % BN;
     k = tb ? e % BKB : e / BN;
   };
-  auto load = [&](const GemmSeg &sg, int k0) {
-    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
-    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
+  // Per-segment addressing (set up at each segment switch).  Since 256 is a
+  // multiple of the slice depth and of BN, element i of this thread sits at
+  // an affine position: offset off0 + i * soff (plus the slice's k0 term),
+  // k = kt + i * sk and row (column) r0 + i * sr, with the steps uniform.
+  __amdgpu_buffer_rsrc_t ar, br;
+  uint32_t a_off0 = 0, b_off0 = 0, a_soff = 0, b_soff = 0, bstep = 1;
+  int a_kt = 0, b_kt = 0, b_sk = 0, a_r0 = 0, b_r0 = 0, b_sr = 0, segK = 0;
+  auto setup = [&](const GemmSeg &sg) {
+    ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + sg.a_off);
+    br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
     const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
     const bool tb = sg.flags & SEG_TRANS_B;
+    segK = sg.K;
+    // A element e = tid + 256 i: (m, k) = (e / BKB, e % BKB)
+    a_r0 = tl.m0 + tid / BKB;
+    a_kt = tid % BKB;
+    a_off0 = (uint32_t)a_r0 * lda + (uint32_t)a_kt;
+    a_soff = (uint32_t)(256 / BKB) * lda;
+    // B element e: (k, n) = (e / BN, e % BN), or (e % BKB, e / BKB) when transposed
+    const int bn = tb ? tid / BKB : tid % BN, bk = tb ? tid % BKB : tid / BN;
+    b_r0 = tl.n0 + bn;
+    b_kt = bk;
+    b_sr = tb ? 256 / BKB : 0;
+    b_sk = tb ? 0 : 256 / BN;
+    b_off0 = tb ? (uint32_t)b_r0 * ldb + (uint32_t)bk : (uint32_t)bk * ldb + (uint32_t)b_r0;
+    b_soff = (uint32_t)(tb ? 256 / BKB : 256 / BN) * ldb;
+    bstep = tb ? 1u : ldb;
+  };
+  auto load = [&](int k0) {
+    const uint32_t kb = (uint32_t)k0 * bstep;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256, m = e / BKB, k = e % BKB;
-      const uint32_t gm = tl.m0 + m, gk = k0 + k;
-      ra[i] = load_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+      const bool ok = a_r0 + i * (256 / BKB) < M && a_kt + k0 < segK;
+      ra[i] = load_as_f32<float>(ar, ok ? a_off0 + (uint32_t)i * a_soff + (uint32_t)k0 : kOob);
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int k, n;
-      b_kn(i, tb, k, n);
-      const uint32_t gn = tl.n0 + n, gk = k0 + k;
-      rb[i] = load_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+      const bool ok = b_r0 + i * b_sr < N && b_kt + i * b_sk + k0 < segK;
+      rb[i] = load_as_f32<TB>(br, ok ? b_off0 + (uint32_t)i * b_soff + kb : kOob);
     }
   };
   auto store = [&](int buf, uint8_t flags) {
@@ -335,7 +383,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_frag_kernel(const GemmTile *__r
   if (s < s_end) {
     GemmSeg cur = segs[s];
     int k0 = 0, buf = 0;
-    load(cur, 0);
+    setup(cur);
+    load(0);
     store(0, cur.flags);
     __syncthreads();
     while (true) {
@@ -348,7 +397,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_frag_kernel(const GemmTile *__r
         if (ns < s_end) nxt = segs[ns];
       }
       const bool more = ns < s_end;
-      if (more) load(nxt, nk);
+      if (more) {
+        if (ns != s) setup(nxt);
+        load(nk);
+      }
       step(buf);
       if (!more) break;
       store(buf ^ 1, nxt.flags);
@@ -616,23 +668,45 @@ __global__ __launch_bounds__(256) void gemm_bf16_cplx_kernel(const GemmTile *__r
     n = tb ? e / BKB : e % BN;
     k = tb ? e % BKB : e / BN;
   };
-  auto load = [&](const GemmSeg &sg, int k0) {
-    const __amdgpu_buffer_rsrc_t ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + 2 * sg.a_off);
-    const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + 2 * sg.b_off);
+  // Per-segment addressing (set up at each segment switch).  Since 256 is a
+  // multiple of the slice depth and of BN, element i of this thread sits at
+  // an affine position: offset off0 + i * soff (plus the slice's k0 term),
+  // k = kt + i * sk and row (column) r0 + i * sr, with the steps uniform.
+  __amdgpu_buffer_rsrc_t ar, br;
+  uint32_t a_off0 = 0, b_off0 = 0, a_soff = 0, b_soff = 0, bstep = 1;
+  int a_kt = 0, b_kt = 0, b_sk = 0, a_r0 = 0, b_r0 = 0, b_sr = 0, segK = 0;
+  auto setup = [&](const GemmSeg &sg) {
+    ar = make_rsrc(reinterpret_cast<const float *>(slots.p[sg.a_slot]) + 2 * sg.a_off);
+    br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + 2 * sg.b_off);
     const uint32_t lda = (uint32_t)sg.lda, ldb = (uint32_t)sg.ldb;
     const bool tb = sg.flags & SEG_TRANS_B;
+    segK = sg.K;
+    // A element e = tid + 256 i: (m, k) = (e / BKB, e % BKB)
+    a_r0 = tl.m0 + tid / BKB;
+    a_kt = tid % BKB;
+    a_off0 = (uint32_t)a_r0 * lda + (uint32_t)a_kt;
+    a_soff = (uint32_t)(256 / BKB) * lda;
+    // B element e: (k, n) = (e / BN, e % BN), or (e % BKB, e / BKB) when transposed
+    const int bn = tb ? tid / BKB : tid % BN, bk = tb ? tid % BKB : tid / BN;
+    b_r0 = tl.n0 + bn;
+    b_kt = bk;
+    b_sr = tb ? 256 / BKB : 0;
+    b_sk = tb ? 0 : 256 / BN;
+    b_off0 = tb ? (uint32_t)b_r0 * ldb + (uint32_t)bk : (uint32_t)bk * ldb + (uint32_t)b_r0;
+    b_soff = (uint32_t)(tb ? 256 / BKB : 256 / BN) * ldb;
+    bstep = tb ? 1u : ldb;
+  };
+  auto load = [&](int k0) {
+    const uint32_t kb = (uint32_t)k0 * bstep;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      const int e = tid + i * 256, m = e / BKB, k = e % BKB;
-      const uint32_t gm = tl.m0 + m, gk = k0 + k;
-      ra[i] = load_cplx_as_f32<float>(ar, ((int)gm < M && (int)gk < sg.K) ? gm * lda + gk : kOob);
+      const bool ok = a_r0 + i * (256 / BKB) < M && a_kt + k0 < segK;
+      ra[i] = load_cplx_as_f32<float>(ar, ok ? a_off0 + (uint32_t)i * a_soff + (uint32_t)k0 : kOob);
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int k, n;
-      b_kn(i, tb, k, n);
-      const uint32_t gn = tl.n0 + n, gk = k0 + k;
-      rb[i] = load_cplx_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+      const bool ok = b_r0 + i * b_sr < N && b_kt + i * b_sk + k0 < segK;
+      rb[i] = load_cplx_as_f32<TB>(br, ok ? b_off0 + (uint32_t)i * b_soff + kb : kOob);
     }
   };
   auto store = [&](int buf, uint8_t flags) {
@@ -700,7 +774,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_cplx_kernel(const GemmTile *__r
   if (s < s_end) {
     GemmSeg cur = segs[s];
     int k0 = 0, buf = 0;
-    load(cur, 0);
+    setup(cur);
+    load(0);
     store(0, cur.flags);
     __syncthreads();
     while (true) {
@@ -713,7 +788,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_cplx_kernel(const GemmTile *__r
         if (ns < s_end) nxt = segs[ns];
       }
       const bool more = ns < s_end;
-      if (more) load(nxt, nk);
+      if (more) {
+        if (ns != s) setup(nxt);
+        load(nk);
+      }
       step(buf);
       if (!more) break;
       store(buf ^ 1, nxt.flags);
