@@ -59,6 +59,14 @@ class Context {
   Context(const Context &) = delete;
   Context &operator=(const Context &) = delete;
   qlg_ctx get() const { return h_; }
+  // the device block cache of plan workspaces: return it to the driver, or
+  // read {cached bytes, hipMalloc calls, reuses, hipFree calls}
+  void ReleaseCached() { Check(qlg_ctx_release_cached(h_), h_); }
+  std::array<int64_t, 4> MemoryStats() const {
+    std::array<int64_t, 4> st{};
+    Check(qlg_ctx_memory_stats(h_, st.data()), h_);
+    return st;
+  }
   static Context &Default() {
     static Context c(0);
     return c;

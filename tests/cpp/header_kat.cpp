@@ -107,5 +107,10 @@ int main(int argc, char **argv) {
   TridiagGsSolver({2.0, 1.0, 3.0}, {0.5, 0.25}, 3, e, vec, 'V');
   std::printf("TRIDIAG %.15f %.15f\n", e, vec[0]);
   delete[] vec;
+  // the plans above took their workspaces from the context's block cache
+  auto &ctx = qlmps_amd::Context::Default();
+  const auto st = ctx.MemoryStats();
+  ctx.ReleaseCached();
+  std::printf("CACHE %lld %lld %lld\n", (long long)st[1], (long long)st[2], (long long)ctx.MemoryStats()[0]);
   return 0;
 }

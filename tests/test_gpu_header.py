@@ -48,3 +48,8 @@ def test_header_layer_lanczos_kat():
     tri = [l for l in lines if l[0] == "TRIDIAG"][0]
     e3, v3 = TridiagGsSolver([2.0, 1.0, 3.0], [0.5, 0.25], 3, "V")
     assert abs(float(tri[1]) - e3) < 1e-14 and abs(abs(float(tri[2])) - abs(v3[0])) < 1e-14
+    # the three plans drew their workspaces from the context's block cache, and
+    # releasing the cache empties it
+    cache = [l for l in lines if l[0] == "CACHE"][0]
+    mallocs, reuses, cached_after = int(cache[1]), int(cache[2]), int(cache[3])
+    assert mallocs > 0 and reuses > 0 and cached_after == 0
