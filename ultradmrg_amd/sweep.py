@@ -374,23 +374,37 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
         Sinv = torch.where(S > 0, 1.0 / torch.where(S > 0, S, torch.ones_like(S)), torch.zeros_like(S))
         P = R * Sinv[..., None].to(dt)                   # unit rows; zero rows for zero values
         S = S * scales[b][:, None]                       # back to the tensor's scale
-        # Orthogonality between rows degrades as eps s_max^2 / (s_i s_j), ~1e-4
-        # at the noise threshold: Newton-Schulz steps toward the polar factor,
-        # P <- 1.5 P - 0.5 (P P^H) P, restore it quadratically.  The
-        # reconstruction moves by at most ~eps s_max^2 / s.  Zero rows stay zero.
-        mask = (S > 0).to(dt)
-        for _ in range(8):
-            PP = P @ P.mH
-            odev = float((PP - torch.diag_embed(mask)).abs().max())
-            if odev < 1e-13:
-                break
-            if odev > 0.5:
-                raise RuntimeError(f"block SVD: singular vectors too far from orthonormal ({odev:.2e})")
-            P = 1.5 * P - 0.5 * (PP @ P)
         per[b] = (E, S, P)
         svals.append(S)
+    # Orthogonality between rows degrades as eps s_max^2 / (s_i s_j), ~1e-4 at
+    # the noise threshold: Newton-Schulz steps toward the polar factor,
+    # P <- 1.5 P - 0.5 (P P^H) P, restore it quadratically.  The reconstruction
+    # moves by at most ~eps s_max^2 / s.  Zero rows stay zero.  The buckets run
+    # in lock step, one host check per round for all of them; each stops at
+    # its own deviation < 1e-13.
+    masks = {b: (per[b][1] > 0).to(dt) for b in st.buckets}
+    active = list(st.buckets)
+    for _ in range(8):
+        if not active:
+            break
+        PPs = {b: per[b][2] @ per[b][2].mH for b in active}
+        devs = torch.stack([(PPs[b] - torch.diag_embed(masks[b])).abs().amax() for b in active]).tolist()
+        still = []
+        for b, odev in zip(active, devs):
+            if odev < 1e-13:
+                continue
+            if odev > 0.5:
+                raise RuntimeError(f"block SVD: singular vectors too far from orthonormal ({odev:.2e})")
+            E, S, P = per[b]
+            per[b] = (E, S, 1.5 * P - 0.5 * (PPs[b] @ P))
+            still.append(b)
+        active = still
     # one host transfer for the truncation decision
-    sv_all = [s.double().cpu().numpy() for s in svals]
+    flat_sv = torch.cat([s_.reshape(-1) for s_ in svals]).double().cpu().numpy()
+    sv_all, pos = [], 0
+    for s_ in svals:
+        sv_all.append(flat_sv[pos:pos + s_.numel()].reshape(s_.shape))
+        pos += s_.numel()
     sv, owner, sv_host = [], [], {}
     for (b, sis), arr in zip(st.buckets.items(), sv_all):
         for j, si in enumerate(sis):
