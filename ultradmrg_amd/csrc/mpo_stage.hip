@@ -125,10 +125,14 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
         As[0][k][m] = ok ? Cm[ix] : T(0);
       }
     }
-    // B fragments of the whole chunk: lane holds X[row kc + 4 s + lane / 16][its column j]
+    // B fragments of the whole chunk: lane holds X[row kc + 4 s + lane / 16][its column j].
+    // k-steps that start at or past K are skipped (wave-uniform), which saves
+    // their load instructions and MFMAs: K_in is short, so most of a chunk's
+    // steps are empty
     T rb[NP][KS][FN];
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
+      if (kc + 4 * st >= K) break;
       const int gk = kc + 4 * st + (lane >> 4);
       const bool kok = gk < K;
       const RowDesc rd = K <= kRowWin ? rin[kok ? gk : 0] : gin[kok ? gk : 0];
@@ -151,6 +155,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
     __syncthreads();  // C chunk staged
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
+      if (kc + 4 * st >= K) break;
       T af[NP][FM];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
