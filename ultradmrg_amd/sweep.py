@@ -83,43 +83,54 @@ class _SvdStructure:
 
     def __init__(self, t: BlockTensor, ldims: int, ldiv):
         left, right = t.axes[:ldims], t.axes[ldims:]
-        sgn = [-1 if ax.dir == OUT else 1 for ax in left]
-        groups: Dict[tuple, dict] = {}
-        for i, c in enumerate(t.coords):
-            q = list(ldiv)
-            for sg, ax, sc in zip(sgn, left, c[:ldims]):
-                for j, v in enumerate(ax.qns[sc]):
-                    q[j] += sg * v
-            g = groups.setdefault(tuple(q), {"rows": {}, "cols": {}, "blocks": []})
-            lk, rk = c[:ldims], c[ldims:]
-            if lk not in g["rows"]:
-                nr = 1
-                for ax, sc in zip(left, lk):
-                    nr *= ax.degs[sc]
-                g["rows"][lk] = nr
-            if rk not in g["cols"]:
-                nc = 1
-                for ax, sc in zip(right, rk):
-                    nc *= ax.degs[sc]
-                g["cols"][rk] = nc
-            g["blocks"].append((i, lk, rk))
-        self.qs = sorted(groups)
-        self.sect = []
-        for q in self.qs:
-            g = groups[q]
-            rks, cks = sorted(g["rows"]), sorted(g["cols"])
-            ro, co, acc = {}, {}, 0
-            for k in rks:
-                ro[k] = acc
-                acc += g["rows"][k]
-            R = acc
-            acc = 0
-            for k in cks:
-                co[k] = acc
-                acc += g["cols"][k]
-            Cn = acc
-            self.sect.append(dict(q=q, rows=g["rows"], cols=g["cols"], rks=rks, cks=cks, ro=ro, co=co, R=R, C=Cn,
-                                  tr=R > Cn, n=min(R, Cn), m=max(R, Cn), blocks=g["blocks"]))
+        nb, qlen = len(t.coords), len(ldiv)
+        cs = np.asarray(t.coords, dtype=np.int64).reshape(nb, t.rank)
+        # per block: left charge (the sector), row / column extents, and the
+        # left / right coordinates as mixed-radix keys (their integer order is
+        # the tuples' lexicographic order)
+        q = np.tile(np.asarray(ldiv, dtype=np.int64), (nb, 1))
+        nr, nc = np.ones(nb, np.int64), np.ones(nb, np.int64)
+        lkey, rkey = np.zeros(nb, np.int64), np.zeros(nb, np.int64)
+        lrad = rrad = 1
+        for k, ax in enumerate(t.axes):
+            col = cs[:, k]
+            deg = np.asarray(ax.degs, dtype=np.int64)[col]
+            if k < ldims:
+                sg = -1 if ax.dir == OUT else 1
+                q += sg * np.asarray(ax.qns, dtype=np.int64).reshape(ax.nsect, qlen)[col]
+                nr *= deg
+                lkey, lrad = lkey * ax.nsect + col, lrad * ax.nsect
+            else:
+                nc *= deg
+                rkey, rrad = rkey * ax.nsect + col, rrad * ax.nsect
+        qs, gid = np.unique(q, axis=0, return_inverse=True)
+        gid = gid.reshape(-1)
+        ng = len(qs)
+
+        def offsets(key, rad, ext):
+            # distinct (sector, key) pairs in sorted order; each pair's offset
+            # within its sector (exclusive prefix sum of extents) and each
+            # sector's total extent
+            pk, first, inv = np.unique(gid * rad + key, return_index=True, return_inverse=True)
+            pg, pext = pk // rad, ext[first]
+            ex = np.cumsum(pext) - pext
+            start = np.searchsorted(pg, np.arange(ng))
+            return first, pg, pext, ex - ex[start][pg], np.add.reduceat(pext, start), inv.reshape(-1)
+
+        rfirst, rg, rext, roff, Rs, rinv = offsets(lkey, lrad, nr)
+        cfirst, cg, cext, coff, Cs, cinv = offsets(rkey, rrad, nc)
+        self.qs = [tuple(int(v) for v in x) for x in qs]
+        self.sect = [dict(q=qq, rows={}, cols={}, rks=[], cks=[], ro={}, co={}, R=int(Rs[g]), C=int(Cs[g]),
+                          tr=bool(Rs[g] > Cs[g]), n=int(min(Rs[g], Cs[g])), m=int(max(Rs[g], Cs[g])))
+                     for g, qq in enumerate(self.qs)]
+        for i, g, e, o in zip(rfirst.tolist(), rg.tolist(), rext.tolist(), roff.tolist()):
+            sc, lk = self.sect[g], t.coords[i][:ldims]
+            sc["rks"].append(lk)
+            sc["rows"][lk], sc["ro"][lk] = e, o
+        for i, g, e, o in zip(cfirst.tolist(), cg.tolist(), cext.tolist(), coff.tolist()):
+            sc, rk = self.sect[g], t.coords[i][ldims:]
+            sc["cks"].append(rk)
+            sc["cols"][rk], sc["co"][rk] = e, o
         # buckets of sectors by padded short side; slot j of bucket b.  A
         # bucket is padded only to its largest member (a multiple of 16), not
         # to the bucket's bound: the eigensolver's time grows with the order.
@@ -135,20 +146,20 @@ class _SvdStructure:
                      for b, sis in self.buckets.items()}
         # per-block table: data offset, rows, cols, batch base, row/col origin in
         # the sector matrix, transposed flag, padded long side
-        nb = len(t.coords)
-        tab = np.zeros((nb, 8), dtype=np.int64)
+        sbase = np.zeros(ng, np.int64)
+        smp = np.zeros(ng, np.int64)
         base = 0
         self.bucket_base = {}
         for b, sis in self.buckets.items():
             mp = self.mpad[b]
             self.bucket_base[b] = base
             for j, si in enumerate(sis):
-                sc = self.sect[si]
-                sbase = base + j * b * mp
-                for (i, lk, rk) in sc["blocks"]:
-                    tab[i] = (t.offsets[i], sc["rows"][lk], sc["cols"][rk], sbase, sc["ro"][lk], sc["co"][rk],
-                              int(sc["tr"]), mp)
+                sbase[si] = base + j * b * mp
+                smp[si] = mp
             base += len(sis) * b * mp
+        tr = np.array([sc["tr"] for sc in self.sect], dtype=np.int64)
+        tab = np.stack([np.asarray(t.offsets, dtype=np.int64).reshape(nb), nr, nc, sbase[gid], roff[rinv],
+                        coff[cinv], tr[gid], smp[gid]], axis=1)
         self.total = base
         dev = t.data.device
         sizes = tab[:, 1] * tab[:, 2]
