@@ -326,6 +326,22 @@ def _gather_cols(src: torch.Tensor, starts, rows, cols, ld, dev) -> torch.Tensor
     return src[T[0][blk] + r * T[2][blk] + (loc - r * c)]
 
 
+def _gather_strided(src: torch.Tensor, tab: np.ndarray, dev) -> torch.Tensor:
+    """Concatenation of the [rows x cols] matrices (row-major) whose element
+    (r, c) is src[base + r * rs + c * cs], one per row (rows, cols, base, rs,
+    cs) of ``tab`` (one device gather)."""
+    size = tab[:, 0] * tab[:, 1]
+    total = int(size.sum())
+    if total == 0:
+        return torch.zeros(1, dtype=src.dtype, device=dev)
+    T = torch.from_numpy(np.ascontiguousarray(np.concatenate([tab, (np.cumsum(size) - size)[:, None]], axis=1).T)).to(dev)
+    blk = torch.repeat_interleave(torch.arange(len(tab), device=dev), T[0] * T[1], output_size=total)
+    loc = torch.arange(total, device=dev) - T[5][blk]
+    c = T[1][blk]
+    r = torch.div(loc, c, rounding_mode="floor")
+    return src[T[2][blk] + r * T[3][blk] + (loc - r * c) * T[4][blk]]
+
+
 def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dmax: int) -> SVDResult:
     """Block SVD with truncation, through FP64 eigen-decompositions of each
     sector matrix's Gram matrix on its shorter side, batched by size bucket.
@@ -445,39 +461,58 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     for b, sis in st.buckets.items():
         for j, si in enumerate(sis):
             slot[si] = (b, j)
-    # Per kept sector, U (rows x k) and V^H (k x cols) are made contiguous in two
-    # flat buffers; the blocks of u and vt are then cut out of them by one
-    # device gather each (u blocks are row ranges of U, vt blocks column ranges
-    # of V^H), and s is scattered onto its diagonal blocks.
-    Ulist, Vlist, Slist = [], [], []
+    # Per kept sector, U (R x k) and V^H (k x C), row-major, are gathered from
+    # the buckets' E and P into two flat buffers (one device gather each), and
+    # its k singular values into a third; the blocks of u and vt are then cut
+    # out of them by one device gather each (u blocks are row ranges of U, vt
+    # blocks column ranges of V^H), and s is scattered onto its diagonal blocks.
+    #   X = M (R <= C):   U = E[:n, :k],        V^H = P[:k, :m]
+    #   X = M^T (R > C):  E = conj(V), so V^H = E[:n, :k]^T and U = M conj(E) / S = P[:k, :m]^T
+    parts, sparts, ebase, pbase, sbase_ = [], [], {}, {}, {}
+    off = soff = 0
+    for b in st.buckets:
+        E, S, P = per[b]
+        ebase[b], pbase[b], sbase_[b] = off, off + E.numel(), soff
+        parts += [E.reshape(-1), P.reshape(-1)]
+        sparts.append(S.reshape(-1))
+        off += E.numel() + P.numel()
+        soff += S.numel()
+    utab = np.zeros((len(kept_si), 5), dtype=np.int64)
+    vtab = np.zeros((len(kept_si), 5), dtype=np.int64)
+    stab = np.zeros((len(kept_si), 5), dtype=np.int64)
     ubase, vbase, ks = {}, {}, {}
     uoff = voff = 0
     for mi, si in enumerate(kept_si):
         sc, k = st.sect[si], int(keep[si])
         b, j = slot[si]
-        E, S, P = per[b]
-        n, m = sc["n"], sc["m"]
-        Sk = S[j, :k]
-        if sc["tr"]:     # X = M^T: E = conj(V), so V^H = E^T and U = M conj(E) / S = P^T
-            U = P[j, :k, :m].mT
-            Vh = E[j, :n, :k].mT
+        n, m, mp = sc["n"], sc["m"], st.mpad[b]
+        eb, pb = ebase[b] + j * b * b, pbase[b] + j * b * mp
+        if sc["tr"]:
+            utab[mi], vtab[mi] = (m, k, pb, 1, mp), (k, n, eb, 1, b)
         else:
-            U = E[j, :n, :k]
-            Vh = P[j, :k, :m]
-        if sv_host[si][k - 1] == 0.0:
-            bad = Sk == 0
-            if sc["tr"]:
-                U = _complete_rows(U.mT.contiguous(), bad).mT
-            else:
-                Vh = _complete_rows(Vh.contiguous(), bad)
-        Ulist.append(U.reshape(-1))
-        Vlist.append(Vh.reshape(-1))
-        Slist.append(Sk)
+            utab[mi], vtab[mi] = (n, k, eb, b, 1), (k, m, pb, mp, 1)
+        stab[mi] = (1, k, sbase_[b] + j * b, 0, 1)
         ubase[si], vbase[si], ks[si] = uoff, voff, k
         uoff += sc["R"] * k
         voff += k * sc["C"]
-    Uflat = torch.cat(Ulist).to(dt)
-    Vflat = torch.cat(Vlist).to(dt)
+    src = torch.cat(parts)
+    Uflat = _gather_strided(src, utab, dev).to(dt)
+    Vflat = _gather_strided(src, vtab, dev).to(dt)
+    Sflat = _gather_strided(torch.cat(sparts), stab, dev)
+    for si in kept_si:
+        k = ks[si]
+        if sv_host[si][k - 1] != 0.0:
+            continue
+        # kept zero singular values: complete their singular vectors
+        sc = st.sect[si]
+        b, j = slot[si]
+        bad = per[b][1][j, :k] == 0
+        if sc["tr"]:
+            U = Uflat[ubase[si]:ubase[si] + sc["R"] * k].view(sc["R"], k)
+            U.copy_(_complete_rows(U.mT.contiguous(), bad).mT)
+        else:
+            Vh = Vflat[vbase[si]:vbase[si] + k * sc["C"]].view(k, sc["C"])
+            Vh.copy_(_complete_rows(Vh.contiguous(), bad))
     # u blocks (lk, mi): rows [ro, ro + rows) of U -> one contiguous run each
     ublk = sorted((lk + (mi,), si, lk) for mi, si in enumerate(kept_si) for lk in st.sect[si]["rks"])
     u_start = [ubase[si] + st.sect[si]["ro"][lk] * ks[si] for _, si, lk in ublk]
@@ -495,7 +530,7 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     sdata = torch.zeros(max(sum(k * k for k in kk), 1), dtype=dt, device=dev)
     sb = np.concatenate([[0], np.cumsum([k * k for k in kk])[:-1]]).astype(np.int64)
     diag_pos = torch.from_numpy(np.concatenate([sb[i] + np.arange(k) * (k + 1) for i, k in enumerate(kk)])).to(dev)
-    sdata[diag_pos] = torch.cat(Slist).to(dt)
+    sdata[diag_pos] = Sflat.to(dt)
     s = BlockTensor([mid_in, mid_out], [(mi, mi) for mi in range(len(kept_si))], sdata)
     pr = np.sort(sv)[::-1][:D] ** 2
     pr = pr / pr.sum()
