@@ -923,7 +923,7 @@ void Plan::autotune(const double *in, double *out) {
           g.make_tiles(sh.first, sh.second);
           g.upload(stream);
           double ms_min = 1e300;
-          for (int rep = 0; rep < 2; ++rep) {
+          for (int rep = 0; rep < 3; ++rep) {   // one warm-up, best of two timed launches
             QLG_HIP(hipEventRecord(e0, stream));
             launch_stage(i, s);
             QLG_HIP(hipEventRecord(e1, stream));
