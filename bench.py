@@ -348,8 +348,8 @@ def main():
     ap.add_argument("--no-chain", action="store_true", help="skip the L=100 Heisenberg ramp")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: the matvec is not sharded across GPUs in this build "
-              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: this library does not shard the matvec across GPUs; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()

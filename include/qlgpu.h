@@ -35,7 +35,7 @@ extern "C" {
 #define QLG_ERR_INVALID 1      /* malformed descriptor / argument          */
 #define QLG_ERR_HIP 2          /* a HIP runtime call failed                */
 #define QLG_ERR_ALLOC 3        /* device allocation failed                 */
-#define QLG_ERR_UNSUPPORTED 4  /* valid request this build does not handle */
+#define QLG_ERR_UNSUPPORTED 4  /* valid request this library does not handle */
 #define QLG_ERR_NUMERIC 5      /* tridiagonal eigen-solver did not converge*/
 
 /* ---- element / arithmetic types ---------------------------------------- */

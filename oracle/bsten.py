@@ -3,7 +3,7 @@ reference's hot path relies on.
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product (``ultradmrg_amd``,
 ``include/``) may import this module; only ``tests/``, ``__graft_entry__.smoke``
-and the ``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker.
+and the ``cpu_baseline`` leg of ``bench.py`` use it, and only to check results.
 
 The reference delegates all tensor arithmetic to the external, un-vendored
 TensorToolkit (qlten).  What is restated here is the behaviour the reference's

@@ -12,7 +12,7 @@ Reference lines followed:
 * ``DirectStateInitMps``                   ``one_dim_tn/mps/finite_mps/finite_mps_init.h:213-262``
 * two-site TDVP sweep                      ``algorithm/tdvp/two_site_update_finite_tdvp_impl.h:204-531``
 
-The block SVD truncation rule is this build's specification (qlten's rule is
+The block SVD truncation rule is this package's specification (qlten's rule is
 not in the reference): singular values of all charge sectors are pooled and
 sorted; D is the smallest count whose discarded weight sum(s^2)/total is
 <= trunc_err, clamped to [Dmin, Dmax]; the reported truncation error is the

@@ -14,7 +14,7 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/bsten.py``).
 * MPO tensors are ``W[wl(IN), s_in(IN), s_out(OUT), wr(OUT)]`` with zero
   divergence and one-dimensional zero-charge boundary bonds
   (``one_dim_tn/mpo/mpogen/mpogen_impl.h:443-504``).  The finite-state
-  automaton here is this build's own: one ``start`` and one ``end`` state per
+  automaton here is this package's own: one ``start`` and one ``end`` state per
   bond plus one state per open (first site, first operator, string) prefix.
   It realises the same Hamiltonian as the reference's symbolic generator; the
   bond dimension may differ, so FLOP counts are stated per MPO.

@@ -3,7 +3,7 @@
 The reference dumps site tensors one file per site (ten_vec.h:61-105, names
 mps_ten<i> from consts.h:37) and stops a DMRG run on signals
 (dmrg_impl.h:478-502).  Its on-disk format is qlten's, which is not part of the
-reference sources, so these tests pin this build's own format: exact round trips
+reference sources, so these tests pin this package's own format: exact round trips
 of real and complex tensors, detection of corruption and truncation, the
 manifest written last, and the signal-to-flag mapping."""
 import json

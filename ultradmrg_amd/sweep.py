@@ -14,7 +14,7 @@ Everything stays on the GPU: contractions and environment updates go through
 the library's planners, the Krylov solve through ``qlg_lanczos_gs``, and the
 per-sector SVDs through batched FP64 eigen-decompositions of the Gram matrix
 on each sector matrix's smaller side (torch / rocSOLVER).  The truncation
-rule is this build's: pooled singular values, smallest D whose discarded
+rule is this package's: pooled singular values, smallest D whose discarded
 weight sum(s^2)/total <= trunc_err, clamped to [Dmin, Dmax].
 """
 from __future__ import annotations
@@ -346,7 +346,7 @@ def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dma
     """Block SVD with truncation, through FP64 eigen-decompositions of each
     sector matrix's Gram matrix on its shorter side, batched by size bucket.
 
-    Truncation (this build's rule): pool the singular values of all sectors,
+    Truncation (this package's rule): pool the singular values of all sectors,
     keep the smallest D whose discarded weight sum(s^2)/total <= trunc_err,
     clamped to [Dmin, Dmax].  The Gram route resolves singular values to an
     absolute ~sqrt(n eps) of the sector's largest, so values at or below 1e-6
