@@ -356,14 +356,12 @@ def main():
         return
 
     import torch
-    from tests.workloads import synthetic_eff_ham
-    from tests.convert import to_device
     from ultradmrg_amd.solver import EffectiveHamiltonian
+    from ultradmrg_amd.workloads import synthetic_eff_ham
 
-    eff, psi = synthetic_eff_ham("hubbard", a.D, seed=1003)
-    d_eff = [to_device(t) for t in eff]
-    x = to_device(psi)
+    d_eff, x = synthetic_eff_ham("hubbard", a.D, seed=1003)
     H = EffectiveHamiltonian(d_eff, x, precision=a.precision)
+    psi_host = x
     x = H.adopt(x)
     y = H.new_vector(zero=False)
     algo, execf = H.flops()
@@ -446,7 +444,8 @@ def main():
     if not a.no_sweep:
         out["sweep"] = hubbard_sweep_comparison(split=a.split, alt=alt)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(eff, psi)
+        from tests.convert import from_device
+        out["cpu_baseline"] = cpu_baseline([from_device(t) for t in d_eff], from_device(psi_host))
     print(json.dumps(out), flush=True)
 
 

@@ -33,6 +33,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "qlgpu.h"
@@ -428,14 +429,32 @@ BSTensor<Elem> *eff_ham_mul_single_site_state(const std::vector<BSTensor<Elem> *
 // ---------------------------------------------------------------------------
 // solvers
 // ---------------------------------------------------------------------------
-// The function pointer selects nothing beyond what eff_ham's size already
-// fixes (two-site: 4 operators, one-site: 3); it is kept for source
-// compatibility.  The whole Krylov loop runs in the library.
+// The whole Krylov loop runs in the library, so the function pointer must be
+// this layer's own matvec for the operator count given: eff_ham_mul_two_site_state
+// with {L, W_l, W_r, R} or eff_ham_mul_single_site_state with {L, W, R}.  Any
+// other pointer (a caller's own matvec, such as the reference's noise
+// expansion, two_site_update_noise_finite_vmps_impl.h:269-273) cannot be
+// offloaded and is refused with std::invalid_argument; the initial state is
+// then left to the caller.
+template <typename TenT>
+void CheckOffloadableMatvec(const std::vector<TenT *> &rpeff_ham,
+                            TenT *(*eff_ham_mul_state)(const std::vector<TenT *> &, TenT *), const char *who) {
+  using Elem = typename std::remove_const<
+      typename std::remove_pointer<decltype(std::declval<TenT &>().DeviceData())>::type>::type;
+  static_assert(std::is_same<TenT, BSTensor<Elem>>::value, "the solver runs on qlmps_amd::BSTensor");
+  const bool two = eff_ham_mul_state == &eff_ham_mul_two_site_state<Elem> && rpeff_ham.size() == 4;
+  const bool one = eff_ham_mul_state == &eff_ham_mul_single_site_state<Elem> && rpeff_ham.size() == 3;
+  if (!two && !one)
+    throw std::invalid_argument(std::string(who) +
+                                ": the matvec must be eff_ham_mul_two_site_state (4 operators) or "
+                                "eff_ham_mul_single_site_state (3 operators); other matvecs cannot run in the library");
+}
+
 template <typename TenT>
 LanczosRes<TenT> LanczosSolver(const std::vector<TenT *> &rpeff_ham, TenT *pinit_state,
                                TenT *(*eff_ham_mul_state)(const std::vector<TenT *> &, TenT *),
                                const LanczosParams &params) {
-  (void)eff_ham_mul_state;
+  CheckOffloadableMatvec(rpeff_ham, eff_ham_mul_state, "LanczosSolver");
   using Elem = typename std::remove_pointer<decltype(pinit_state->DeviceData())>::type;
   EffectiveHamiltonian<typename std::remove_const<Elem>::type> H(rpeff_ham, *pinit_state);
   TenT *work = H.Adopt(*pinit_state);
@@ -476,7 +495,7 @@ template <typename TenT>
 ExpmvRes<TenT> LanczosExpmvSolver(const std::vector<TenT *> &rpeff_ham, TenT *pinit_state,
                                   TenT *(*eff_ham_mul_state)(const std::vector<TenT *> &, TenT *),
                                   const double step_length, const LanczosParams &params) {
-  (void)eff_ham_mul_state;
+  CheckOffloadableMatvec(rpeff_ham, eff_ham_mul_state, "LanczosExpmvSolver");
   using Elem = typename std::remove_pointer<decltype(pinit_state->DeviceData())>::type;
   EffectiveHamiltonian<typename std::remove_const<Elem>::type> H(rpeff_ham, *pinit_state);
   TenT *work = H.Adopt(*pinit_state);

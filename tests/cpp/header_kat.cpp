@@ -66,6 +66,29 @@ int main(int argc, char **argv) {
     std::printf("E %.15f ITERS %zu NORM %.15f\n", res.gs_eng, res.iters, res.gs_vec->Norm());
     delete res.gs_vec;
   }
+  // a matvec of the caller's own (here: a trivial one) cannot run in the library: refused,
+  // and the initial state stays with the caller
+  {
+    auto *init = new DTen({Din, dout, dout, Dout});
+    DTen *(*foreign)(const std::vector<DTen *> &, DTen *) = [](const std::vector<DTen *> &, DTen *s) {
+      return new DTen(*s);
+    };
+    bool refused = false;
+    try {
+      (void)LanczosSolver<DTen>({&lblock, &lsite, &rsite, &rblock}, init, foreign, LanczosParams(1.0E-9));
+    } catch (const std::invalid_argument &) {
+      refused = true;
+    }
+    bool mismatched = false;   // the one-site matvec with four operators
+    try {
+      (void)LanczosSolver<DTen>({&lblock, &lsite, &rsite, &rblock}, init, &eff_ham_mul_single_site_state<double>,
+                                LanczosParams(1.0E-9));
+    } catch (const std::invalid_argument &) {
+      mismatched = true;
+    }
+    delete init;
+    std::printf("FOREIGN %d %d\n", (int)refused, (int)mismatched);
+  }
   // the same operator in the DMRG (super-block) form of lanczos_dmrg_solver_impl.h:61-66:
   // one term group per MPO bond value k, {(L_k, m)} with {(m, R_k)}, L_k[a, a'] = lblock(a', k, a)
   // = M[a'][a] and R_k[b, b'] = rblock(b', k, b) = M[b][b']

@@ -97,3 +97,70 @@ def test_stop_flags():
     finally:
         f.uninstall()
     assert signal.getsignal(signal.SIGINT) is signal.default_int_handler
+
+
+def _scaled(mps, c):
+    from ultradmrg_amd.tensor import BlockTensor
+    return [BlockTensor(t.axes, t.coords, t.data * c, t.offsets) for t in mps]
+
+
+def test_interrupted_second_dump_keeps_the_first(tmp_path, monkeypatch):
+    """A second dump killed part-way (here: after two site files) leaves the first
+    dump readable and unchanged; nothing of the partial dump is read."""
+    mps = [to_device(t, device="cpu") for t in _mps("heisenberg")]
+    d = str(tmp_path / "mps")
+    ck.save_mps(d, mps, {"sweeps_done": 1})
+    real = ck._pack_tensor
+    calls = [0]
+
+    def dying_pack(t):
+        calls[0] += 1
+        if calls[0] == 3:
+            raise KeyboardInterrupt("killed during the dump")
+        return real(t)
+
+    monkeypatch.setattr(ck, "_pack_tensor", dying_pack)
+    with pytest.raises(KeyboardInterrupt):
+        ck.save_mps(d, _scaled(mps, 2.0), {"sweeps_done": 2})
+    monkeypatch.setattr(ck, "_pack_tensor", real)
+    back, man = ck.load_mps(d, "cpu")
+    assert man["sweeps_done"] == 1
+    for a, b in zip(back, mps):
+        _same(a, b)
+    # a later complete dump replaces it and clears the partial directory
+    ck.save_mps(d, _scaled(mps, 2.0), {"sweeps_done": 2})
+    assert not os.path.exists(d + ".partial") and not os.path.exists(d + ".prev")
+    back, man = ck.load_mps(d, "cpu")
+    assert man["sweeps_done"] == 2
+    assert torch.equal(back[0].data, mps[0].data * 2.0)
+
+
+def test_swap_cut_between_renames_reads_previous(tmp_path):
+    """If the swap is cut after the old dump was renamed to .prev and before the
+    new one took its place, the loader reads the previous complete dump."""
+    mps = [to_device(t, device="cpu") for t in _mps("hubbard")]
+    d = str(tmp_path / "mps")
+    ck.save_mps(d, mps, {"sweeps_done": 4})
+    os.rename(d, d + ".prev")
+    back, man = ck.load_mps(d, "cpu")
+    assert man["sweeps_done"] == 4
+    for a, b in zip(back, mps):
+        _same(a, b)
+
+
+def test_mixed_generation_and_bond_mismatch_rejected(tmp_path):
+    """A site file from another dump (size/CRC not in the manifest) and a bond
+    that does not match its neighbour are both refused."""
+    mps = [to_device(t, device="cpu") for t in _mps("heisenberg")]
+    d1, d2 = str(tmp_path / "a"), str(tmp_path / "b")
+    ck.save_mps(d1, mps)
+    ck.save_mps(d2, _scaled(mps, 3.0))
+    import shutil
+    shutil.copy(ck.mps_tensor_name(d2, 2), ck.mps_tensor_name(d1, 2))
+    with pytest.raises(ck.CheckpointError, match="does not belong"):
+        ck.load_mps(d1, "cpu")
+    bad = list(mps)
+    bad[3] = to_device(o_random_mps(o_heis(8), (0,), 4, seed=5)[3], device="cpu")
+    ck.save_mps(d2, bad)
+    with pytest.raises(ck.CheckpointError, match="bond"):
+        ck.load_mps(d2, "cpu")

@@ -1,13 +1,23 @@
-"""Batching form: independent random-field Heisenberg chains solved together.
+"""Batching form: independent random-field Heisenberg chains solved together,
+checked against the oracle at the configuration's own shapes.
 
-The batch plan runs every matvec stage of all chains as one grouped launch,
-and the batched Lanczos solver advances all chains in lock step.  Each chain's
-result must be the one a separate solve gives: the same iteration count,
-energy within 1e-12 relative and the same ground state; the batched matvec
-equals each member's own matvec to 1e-13; with hipGraph capture enabled, a
-second solve replays the captured iterations and reproduces the first."""
+Workload: 32 chains, L ~ U[32, 128], J = 1, fields h_i ~ U(-2, 2), D drawn from
+{128, 256, 512, 768}, seed 1004 (``ultradmrg_amd.chains``); the centre two-site
+effective Hamiltonian of each.  The batch plan runs every matvec stage of all
+chains as one grouped launch; the batched Lanczos advances all chains in lock
+step (hipGraph capture of its iterations optional).
+
+Bars, per chain: the batched matvec equals the oracle's four-step contraction
+on that chain's tensors within 1e-12 (max-abs error over max-abs value); the
+batched Lanczos energy equals the oracle's LanczosSolver on the same inputs
+within 1e-10 relative with the same iteration count; a graph-replayed second
+solve reproduces the first bit for bit."""
+import numpy as np
 import pytest
 import torch
+
+from oracle.lanczos import LanczosParams as OParams, LanczosSolver as OLanczos, eff_ham_mul_two_site_state
+from tests.convert import from_device
 
 pytestmark = pytest.mark.gpu
 
@@ -15,44 +25,77 @@ if torch.cuda.is_available():
     from ultradmrg_amd.chains import random_field_chains
     from ultradmrg_amd.solver import BatchedEffectiveHamiltonian, EffectiveHamiltonian, LanczosParams
 
-
-def _chains():
-    return random_field_chains(5, seed=1004, lmin=8, lmax=16, dims=(8, 16, 24))
-
-
-def test_batched_matvec_equals_members():
-    print("[case] batched matvec, 5 chains", flush=True)
-    chains = _chains()
-    hams = [EffectiveHamiltonian(c.eff_ham, c.psi) for c in chains]
-    batch = BatchedEffectiveHamiltonian(hams)
-    x = batch.pack([c.psi for c in chains])
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    batch.matvec_into(x, y)
-    for h, c, o in zip(hams, chains, batch.offsets):
-        ref = h @ c.psi
-        got = y[o:o + h.size]
-        assert float((got - ref.data[:h.size]).abs().max()) <= 1e-13 * float(ref.data.abs().max())
-    a, _ = batch.flops()
-    assert abs(a - sum(h.flops()[0] for h in hams)) <= 1e-9 * a
+NCHAINS, GROUPS = 32, 4
+PARAMS = (1e-10, 40)
+_STATE = {}
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_batched_lanczos_equals_separate_solves(graphs):
-    print(f"[case] batched lanczos graphs={graphs}", flush=True)
-    chains = _chains()
-    prm = LanczosParams(1e-10, 60)
-    hams = [EffectiveHamiltonian(c.eff_ham, c.psi) for c in chains]
-    single = [h.lanczos(c.psi, prm) for h, c in zip(hams, chains)]
-    batch = BatchedEffectiveHamiltonian(hams, graphs=graphs)
-    runs = [batch.lanczos([c.psi for c in chains], prm) for _ in range(2 if graphs else 1)]
-    for res in runs:
-        for s, b in zip(single, res):
-            assert b.iters == s.iters
-            assert abs(b.gs_eng - s.gs_eng) <= 1e-12 * abs(s.gs_eng)
-            va, vb = s.gs_vec.data[:b.gs_vec.data.numel()], b.gs_vec.data
-            ov = float(torch.dot(va, vb)) / float(torch.linalg.vector_norm(va) * torch.linalg.vector_norm(vb))
-            assert abs(abs(ov) - 1.0) < 1e-10
-    if graphs:
-        for a, b in zip(runs[0], runs[1]):
-            assert a.iters == b.iters and a.gs_eng == b.gs_eng
+def _batch():
+    """Chains, their oracle tensors, the batched matvec output and two batched
+    (graph-captured, then replayed) Lanczos solves, computed once."""
+    if not _STATE:
+        chains = random_field_chains(NCHAINS, seed=1004)
+        hams = [EffectiveHamiltonian(c.eff_ham, c.psi) for c in chains]
+        batch = BatchedEffectiveHamiltonian(hams, graphs=True)
+        x = batch.pack([c.psi for c in chains])
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        batch.matvec_into(x, y)
+        prm = LanczosParams(*PARAMS)
+        runs = [batch.lanczos([c.psi for c in chains], prm) for _ in range(2)]
+        _STATE.update(chains=chains, hams=hams, batch=batch, y=y.cpu(), runs=runs)
+    return _STATE
+
+
+def test_config_shapes():
+    s = _batch()
+    ch = s["chains"]
+    assert len(ch) == NCHAINS
+    assert all(32 <= c.length <= 128 for c in ch)
+    assert {c.D for c in ch} <= {128, 256, 512, 768} and len({c.D for c in ch}) >= 3
+    a, _ = s["batch"].flops()
+    assert abs(a - sum(h.flops()[0] for h in s["hams"])) <= 1e-9 * a
+
+
+@pytest.mark.parametrize("group", range(GROUPS))
+def test_batched_matvec_vs_oracle(group):
+    s = _batch()
+    for i in range(group, NCHAINS, GROUPS):
+        c, h, o = s["chains"][i], s["hams"][i], s["batch"].offsets[i]
+        print(f"[case] batch matvec chain {i} L={c.length} D={c.D}", flush=True)
+        oeff, opsi = [from_device(t) for t in c.eff_ham], from_device(c.psi)
+        ref = eff_ham_mul_two_site_state(oeff, opsi)
+        assert h.layout_coords == sorted(ref.blocks)
+        got = s["y"][o:o + h.size].numpy()
+        refv = np.concatenate([ref.blocks[k].reshape(-1) for k in h.layout_coords])
+        assert np.abs(got - refv).max() <= 1e-12 * np.abs(refv).max()
+
+
+@pytest.mark.parametrize("group", range(GROUPS))
+def test_batched_lanczos_vs_oracle(group):
+    s = _batch()
+    first, second = s["runs"]
+    for i in range(group, NCHAINS, GROUPS):
+        c = s["chains"][i]
+        print(f"[case] batch lanczos chain {i} L={c.length} D={c.D}", flush=True)
+        oeff, opsi = [from_device(t) for t in c.eff_ham], from_device(c.psi)
+        ores = OLanczos(oeff, opsi, eff_ham_mul_two_site_state, OParams(*PARAMS))
+        b = first[i]
+        assert b.iters == ores.iters
+        assert abs(b.gs_eng - ores.gs_eng) <= 1e-10 * abs(ores.gs_eng)
+        assert second[i].iters == b.iters and second[i].gs_eng == b.gs_eng
+        assert torch.equal(second[i].gs_vec.data, b.gs_vec.data)
+
+
+def test_batched_lanczos_equals_separate_solves():
+    s = _batch()
+    prm = LanczosParams(*PARAMS)
+    for i in range(0, NCHAINS, 3):
+        print(f"[case] batch vs separate solve chain {i}", flush=True)
+        h, c, b = s["hams"][i], s["chains"][i], s["runs"][0][i]
+        single = h.lanczos(c.psi, prm)
+        assert b.iters == single.iters
+        assert abs(b.gs_eng - single.gs_eng) <= 1e-12 * abs(single.gs_eng)
+        va, vb = single.gs_vec.data[:b.gs_vec.data.numel()], b.gs_vec.data
+        ov = float(torch.dot(va, vb)) / float(torch.linalg.vector_norm(va) * torch.linalg.vector_norm(vb))
+        assert abs(abs(ov) - 1.0) < 1e-10

@@ -45,6 +45,9 @@ def test_header_layer_lanczos_kat():
     # DMRG-form overload on the same operator: same energy and iteration count as the first case
     assert abs(sb[0][0] - w0) < 1e-8
     assert abs(sb[0][0] - runs[0][0]) <= 1e-12 * abs(w0) and sb[0][1] == runs[0][1]
+    # a matvec that is not the layer's own, or one that does not fit the operator count, is refused
+    foreign = [l for l in lines if l[0] == "FOREIGN"][0]
+    assert foreign[1:] == ["1", "1"]
     tri = [l for l in lines if l[0] == "TRIDIAG"][0]
     e3, v3 = TridiagGsSolver([2.0, 1.0, 3.0], [0.5, 0.25], 3, "V")
     assert abs(float(tri[1]) - e3) < 1e-14 and abs(abs(float(tri[2])) - abs(v3[0])) < 1e-14

@@ -355,12 +355,16 @@ def test_block_svd_parity(case):
 
 
 def test_checkpoint_resume(tmp_path):
-    """A run cut after one sweep and resumed from its dump repeats the uninterrupted
-    run's sweep energies; an emergency stop (SIGTERM, dmrg_impl.h:478-502) inside a
-    sweep dumps the state and a resumed run reaches the same ground-state energy."""
+    """A run cut after one sweep and resumed from its dump (run() with the original
+    parameters continues from sweeps_done) repeats the uninterrupted run's sweep
+    energies and lands on the exact L=12 ground-state energy (exact
+    diagonalisation); an emergency stop (SIGTERM, dmrg_impl.h:478-502) inside a
+    sweep dumps the state, and the resumed run reaches the exact energy too."""
     import os
     import signal
+    from oracle.models import ed_ground_energy, heisenberg_chain as o_heis
     print("[case] checkpoint resume heisenberg L=12 D=64", flush=True)
+    e_exact = ed_ground_energy(o_heis(12), (0,))
     m = M.heisenberg_chain(12)
     mpo = M.build_mpo(m)
     prm = lambda k, d=None: SweepParams(k, 64, 64, 1e-12, LanczosParams(1e-11, 60), verbose=False,
@@ -369,13 +373,19 @@ def test_checkpoint_resume(tmp_path):
     energies = []
     for _ in range(3):
         energies.append(full.sweep())
+    assert abs(energies[-1] - e_exact) < 1e-10 * abs(e_exact)
     d = str(tmp_path / "ck")
     first = TwoSiteVMPS(random_mps(m, (0,), 64, seed=77), mpo, prm(1, d))
     assert abs(first.run() - energies[0]) < 1e-12
-    again = TwoSiteVMPS.resume(d, mpo, prm(2))
+    again = TwoSiteVMPS.resume(d, mpo, prm(3, d))
     assert again.sweeps_done == 1
-    tail = [again.sweep() for _ in range(2)]
+    tail = []
+    orig_sweep = again.sweep
+    again.sweep = lambda: tail.append(orig_sweep()) or tail[-1]
+    e = again.run()
+    assert again.sweeps_done == 3 and len(tail) == 2
     assert max(abs(a - b) for a, b in zip(tail, energies[1:])) < 1e-11
+    assert abs(e - e_exact) < 1e-10 * abs(e_exact)
 
     d2 = str(tmp_path / "ck2")
     cut = TwoSiteVMPS(random_mps(m, (0,), 64, seed=77), mpo, prm(4, d2))
@@ -395,9 +405,10 @@ def test_checkpoint_resume(tmp_path):
     man = json.load(open(os.path.join(d2, "manifest.json")))
     assert man["complete_sweep"] is False and man["sweeps_done"] == 1
     assert man["stopped_at"] == ["r", 3]
-    back = TwoSiteVMPS.resume(d2, mpo, prm(3))
+    back = TwoSiteVMPS.resume(d2, mpo, prm(4))
     e = back.run()
-    assert abs(e - energies[-1]) < 1e-9
+    assert back.sweeps_done == 4
+    assert abs(e - e_exact) < 1e-10 * abs(e_exact)
 
 
 @pytest.mark.parametrize("cplx", [False, True])

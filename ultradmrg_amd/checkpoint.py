@@ -15,7 +15,9 @@ own, versioned format for a block-sparse tensor:
     crc32 u32 of everything before it
 
 All integers are little-endian.  Files are written to a temporary name and
-renamed, so an interrupted dump never leaves a truncated file in place.
+renamed, so an interrupted dump never leaves a truncated file in place; a
+whole-MPS dump is written into a fresh directory and swapped in as a unit
+(``save_mps``).
 """
 from __future__ import annotations
 
@@ -116,24 +118,95 @@ def mps_tensor_name(mps_path: str, i: int) -> str:
     return os.path.join(mps_path, f"mps_ten{i}.qlgt")
 
 
+def _fsync_dir(path: str):
+    fd = os.open(path, os.O_RDONLY)
+    try:
+        os.fsync(fd)
+    finally:
+        os.close(fd)
+
+
 def save_mps(mps_path: str, mps: List[BlockTensor], meta: Optional[dict] = None):
-    """Dump every site tensor, then the manifest (written last: a directory with
-    a manifest always holds a complete MPS)."""
-    os.makedirs(mps_path, exist_ok=True)
+    """Dump every site tensor and a manifest as one unit.
+
+    The dump is written into a fresh sibling directory ``<mps_path>.partial``;
+    the manifest goes last and records a generation id, the number of sites
+    and each file's size and body CRC32 (the file's own trailer).  Only then is the finished directory
+    swapped in: the previous dump is renamed to ``<mps_path>.prev``, the new
+    one to ``<mps_path>``, and ``.prev`` is removed.  A process killed at any
+    point leaves either the previous complete dump or the new one readable
+    (``load_mps`` falls back to ``.prev`` when the swap was cut between its two
+    renames); a half-written dump is never read."""
+    import shutil
+    import uuid
+    mps_path = os.path.abspath(mps_path)
+    parent = os.path.dirname(mps_path)
+    os.makedirs(parent, exist_ok=True)
+    part, prev = mps_path + ".partial", mps_path + ".prev"
+    if os.path.exists(part):
+        shutil.rmtree(part)
+    os.makedirs(part)
+    files = []
     for i, t in enumerate(mps):
-        save_tensor(mps_tensor_name(mps_path, i), t)
-    man = {"format": "qlgt", "version": VERSION, "length": len(mps)}
+        payload = _pack_tensor(t)
+        _atomic_write(mps_tensor_name(part, i), payload)
+        # the file's own trailer: CRC32 of its body (checked again when it is unpacked)
+        files.append({"name": os.path.basename(mps_tensor_name(part, i)), "bytes": len(payload),
+                      "crc32": struct.unpack("<I", payload[-4:])[0]})
+    man = {"format": "qlgt", "version": VERSION, "length": len(mps), "generation": uuid.uuid4().hex,
+           "files": files}
     man.update(meta or {})
-    _atomic_write(os.path.join(mps_path, "manifest.json"), json.dumps(man, indent=1).encode())
+    _atomic_write(os.path.join(part, "manifest.json"), json.dumps(man, indent=1).encode())
+    _fsync_dir(part)
+    if os.path.exists(prev):
+        shutil.rmtree(prev)
+    if os.path.exists(mps_path):
+        os.rename(mps_path, prev)
+    os.rename(part, mps_path)
+    _fsync_dir(parent)
+    if os.path.exists(prev):
+        shutil.rmtree(prev)
+
+
+def _check_bonds(mps: List[BlockTensor]):
+    """Neighbouring site tensors must share their bond: the right bond of site i
+    is the inverse of the left bond of site i+1 (same sectors and degeneracies)."""
+    for i in range(len(mps) - 1):
+        a, b = mps[i].axes[-1], mps[i + 1].axes[0]
+        if a.qns != b.qns or a.degs != b.degs or a.dir != -b.dir:
+            raise CheckpointError(f"MPS checkpoint: bond between sites {i} and {i + 1} does not match")
 
 
 def load_mps(mps_path: str, device="cuda") -> Tuple[List[BlockTensor], dict]:
+    """Read a dump written by ``save_mps``: every file's size and body CRC32
+    must match the manifest (and the body its CRC), and neighbouring sites
+    must share their bonds."""
+    mps_path = os.path.abspath(mps_path)
     man_path = os.path.join(mps_path, "manifest.json")
     if not os.path.exists(man_path):
-        raise CheckpointError(f"no complete MPS checkpoint in {mps_path}")
+        prev = mps_path + ".prev"
+        if os.path.exists(os.path.join(prev, "manifest.json")):
+            mps_path, man_path = prev, os.path.join(prev, "manifest.json")
+        else:
+            raise CheckpointError(f"no complete MPS checkpoint in {mps_path}")
     with open(man_path) as f:
         man = json.load(f)
-    mps = [load_tensor(mps_tensor_name(mps_path, i), device) for i in range(int(man["length"]))]
+    n = int(man["length"])
+    files = man.get("files")
+    if files is None or len(files) != n:
+        raise CheckpointError("MPS checkpoint manifest does not list every site file")
+    mps = []
+    for i in range(n):
+        path = mps_tensor_name(mps_path, i)
+        if not os.path.exists(path):
+            raise CheckpointError(f"MPS checkpoint: site file {i} is missing")
+        with open(path, "rb") as f:
+            buf = f.read()
+        if len(buf) != int(files[i]["bytes"]) or struct.unpack("<I", buf[-4:])[0] != int(files[i]["crc32"]):
+            raise CheckpointError(f"MPS checkpoint: site file {i} does not belong to this dump "
+                                  f"(generation {man.get('generation')})")
+        mps.append(_unpack_tensor(buf, device))
+    _check_bonds(mps)
     return mps, man
 
 

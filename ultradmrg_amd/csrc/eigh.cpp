@@ -8,11 +8,23 @@
 // bound by host-side launch latency, not by the device.  Groups are therefore
 // spread over kLanes lanes, each a host thread with its own stream and rocBLAS
 // handle, forked from and joined back into the context stream with events.
+//
+// Initialisation is serial.  rocBLAS loads its Tensile kernel libraries
+// lazily, on a handle's first GEMM; four lanes making their first call at the
+// same moment from four threads ran that initialisation concurrently, and
+// under a profiler (which slows code-object loading) it failed in every lane
+// ("Could not initialize Tensile host: unordered_map::at"), after which the
+// lanes' dispatches were malformed and the queue aborted.  So the pool calls
+// rocblas_initialize() once per process, and each lane's handle makes its
+// first GEMM and its first syevd on its own stream, one lane after another,
+// before any lane thread exists.
+#include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <climits>
 #include <exception>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -54,6 +66,35 @@ void destroy_eigh_pool(EighPool *p) {
   delete p;
 }
 
+// First use of every lane's handle, serially: one small GEMM (loads the
+// Tensile kernels for the handle) and one small syevd on the lane's stream,
+// each lane drained before the next starts.
+static void warm_up(EighPool &p) {
+  static std::once_flag once;
+  std::call_once(once, [] { rocblas_initialize(); });
+  constexpr int n = 64;
+  DevBuf buf;
+  buf.ensure((size_t)(4 * n * n + 2 * n) * sizeof(double) + (size_t)sizeof(int) + 256);
+  double *A = buf.as<double>(), *B = A + n * n, *C = B + n * n, *S = C + n * n, *w = S + n * n, *e = w + n;
+  int *info = reinterpret_cast<int *>(e + n);
+  std::vector<double> h(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) h[i * n + j] = (i == j) ? 2.0 + i : 1.0 / (1.0 + i + j);
+  QLG_HIP(hipMemcpy(A, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  QLG_HIP(hipMemcpy(B, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  const double one = 1.0, zero = 0.0;
+  for (auto &l : p.lane) {
+    if (rocblas_dgemm(l.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, A, n, B, n, &zero, C, n) !=
+        rocblas_status_success)
+      throw Error(QLG_ERR_HIP, "eigensolver lane warm-up: rocblas_dgemm failed");
+    QLG_HIP(hipMemcpyAsync(S, A, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, l.stream));
+    if (rocsolver_dsyevd(l.handle, rocblas_evect_original, rocblas_fill_lower, n, S, n, w, e, info) !=
+        rocblas_status_success)
+      throw Error(QLG_ERR_HIP, "eigensolver lane warm-up: rocsolver_dsyevd failed");
+    QLG_HIP(hipStreamSynchronize(l.stream));
+  }
+}
+
 static EighPool *pool(Ctx &c) {
   if (c.eig) return c.eig;
   auto *p = new EighPool();
@@ -68,6 +109,7 @@ static EighPool *pool(Ctx &c) {
         throw Error(QLG_ERR_HIP, "rocblas_set_stream failed");
     }
     QLG_HIP(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+    warm_up(*p);
   } catch (...) {
     destroy_eigh_pool(p);
     throw;
@@ -153,7 +195,13 @@ void eigh_batched(Ctx &c, int dtype, int64_t ngroups, const qlg_eigh_group *grou
   }
   for (auto &t : th) t.join();
   for (int j = 0; j < kLanes; ++j)
-    if (err[j]) std::rethrow_exception(err[j]);
+    if (err[j]) {
+      // work already queued on the other lanes writes the caller's buffers and
+      // the pool's workspace: drain every lane before reporting the failure
+      for (int i = 0; i < kLanes; ++i)
+        if (!work[i].empty()) (void)hipStreamSynchronize(p->lane[i].stream);
+      std::rethrow_exception(err[j]);
+    }
   for (int j = 0; j < kLanes; ++j)
     if (!work[j].empty()) QLG_HIP(hipStreamWaitEvent(c.stream, p->lane[j].done, 0));
   QLG_HIP(hipMemcpyAsync(p->host_info, info, (size_t)ni * sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -777,7 +777,8 @@ class TwoSiteVMPS:
         return e0
 
     def run(self) -> float:
-        """``params.sweeps`` sweeps.  With ``checkpoint_dir`` set, the MPS is dumped
+        """Sweeps until ``params.sweeps`` are done in total (a resumed driver
+        continues from its ``sweeps_done``).  With ``checkpoint_dir`` set, the MPS is dumped
         after every sweep; SIGUSR1 / SIGUSR2 stop after the current sweep and
         SIGINT / SIGTERM after the current update (``dmrg_impl.h:478-502``), in
         both cases with the state dumped first."""
@@ -786,7 +787,7 @@ class TwoSiteVMPS:
         flags = StopFlags().install() if self.p.checkpoint_dir else None
         self._flags = flags
         try:
-            for k in range(self.p.sweeps):
+            for k in range(self.sweeps_done, self.p.sweeps):
                 e0 = self.sweep()
                 if e0 is None:        # emergency stop inside the sweep (already dumped)
                     break
@@ -816,10 +817,12 @@ class TwoSiteVMPS:
 
 
 def center_to_zero(mps: List[BlockTensor], start: Optional[int] = None):
-    """Right-canonicalise sites start .. 1 (default N-1 .. 1) by SVD without truncation."""
+    """Right-canonicalise sites start .. 1 (default N-1 .. 1) by SVD without
+    truncation: every singular value is kept (Dmin = the full rank), so bonds
+    keep their dimension and the state is unchanged."""
     n = len(mps)
     for i in range(n - 1 if start is None else start, 0, -1):
-        r = block_svd(mps[i], 1, divergence(mps[i]), 0.0, 1, 10 ** 9)
+        r = block_svd(mps[i], 1, divergence(mps[i]), 0.0, 10 ** 9, 10 ** 9)
         mps[i] = r.vt
         mps[i - 1] = contract(mps[i - 1], contract(r.u, r.s, 1), 1)
 

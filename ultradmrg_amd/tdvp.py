@@ -141,10 +141,12 @@ class TwoSiteTDVP:
         self.stats.step_seconds.append(time.perf_counter() - t0)
 
     def measure(self, op: np.ndarray) -> np.ndarray:
-        """<O_i> for all sites.  Moves the centre to site 0 (the sweep leaves it at site 1)."""
-        center_to_zero(self.mps)
-        self.rebuild_envs()
-        return local_expectations(self.mps, op)
+        """<O_i> for all sites, on a copy of the state with its centre moved to
+        site 0 (the sweep leaves it at site 1); the evolving state and its
+        environments are left as they are."""
+        mps = list(self.mps)
+        center_to_zero(mps)
+        return local_expectations(mps, op)
 
 
 def apply_local_op(mps: List[BlockTensor], op: np.ndarray, site: int):
