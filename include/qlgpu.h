@@ -237,6 +237,25 @@ int qlg_lanczos_gs(qlg_plan plan, void *psi_inout,
                    const qlg_lanczos_params *params, qlg_lanczos_result *res);
 int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau,
                       const qlg_lanczos_params *params, int64_t *iters);
+/* How the two solvers above run.  Each iteration ends on the GPU: an FP64
+ * kernel solves the tridiagonal problem (the lowest eigenvalue of T for
+ * qlg_lanczos_gs; exp(-i*tau*T) e_1 for qlg_lanczos_expmv) and applies the
+ * stop rule, and the host checks the device's stop flag once per chunk of
+ * iterations (kernels queued past the stop return at once).  For
+ * qlg_lanczos_gs the final Ritz vector comes from the same tridiagonal solve
+ * on the host that TridiagGsSolver does; qlg_lanczos_expmv combines the
+ * basis with the device's coefficients.  Krylov dimensions above 2047 (ground
+ * state) or 64 (expmv) run the iteration-by-iteration host form instead.
+ * qlg_plan_use_graphs(plan, 1) also applies to these solvers: each chunk of
+ * iterations is captured as a hipGraph on first use and replayed by later
+ * solves on the same plan with the same parameters.
+ * qlg_plan_set_reorth(plan, passes): full reorthogonalisation of every new
+ * residual against the whole Krylov basis by classical Gram-Schmidt, applied
+ * `passes` times per iteration (0: off, the reference's plain three-term
+ * recurrence, lanczos_vmps_solver_impl.h:173-183; 1 or 2).  The residual norm
+ * b_m becomes the norm after reorthogonalisation.  No counterpart in the
+ * reference; meant for the reduced-precision modes.                        */
+int qlg_plan_set_reorth(qlg_plan plan, int passes);
 
 /* ---- independent chains in one launch ------------------------------------------
  * The batching form of the path: several effective Hamiltonians (each a

@@ -32,7 +32,7 @@ EXPORTS = [
     "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
     "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
     "qlg_linear_combine", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_eigh_batched", "qlg_plan_batch", "qlg_plan_batch_offsets",
-    "qlg_plan_use_graphs", "qlg_lanczos_gs_batch", "qlg_tridiag_gs",
+    "qlg_plan_use_graphs", "qlg_plan_set_reorth", "qlg_lanczos_gs_batch", "qlg_tridiag_gs",
     "qlg_tridiag_expme1",
 ]
 
@@ -123,6 +123,7 @@ def lib():
         "qlg_plan_batch": (C.c_int, [vp, i64, C.POINTER(vp), C.POINTER(vp)]),
         "qlg_plan_batch_offsets": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64)]),
         "qlg_plan_use_graphs": (C.c_int, [vp, C.c_int]),
+        "qlg_plan_set_reorth": (C.c_int, [vp, C.c_int]),
         "qlg_lanczos_gs_batch": (C.c_int, [vp, vp, C.POINTER(LanczosParams), C.POINTER(LanczosResult)]),
         "qlg_tridiag_gs": (C.c_int, [dp, dp, i64, C.c_char, dp, dp]),
         "qlg_tridiag_expme1": (C.c_int, [dp, dp, i64, C.c_double, dp]),

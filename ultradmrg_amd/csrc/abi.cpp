@@ -238,9 +238,20 @@ int qlg_plan_use_graphs(qlg_plan plan, int on) {
   if (!plan) return QLG_ERR_INVALID;
   Plan &p = *plan->p;
   return guarded(p.ctx, [&] {
-    QLG_CHECK(p.batch_off.size() >= 2, "graph capture applies to batch plans");
+    QLG_CHECK(p.kind == PLAN_HEFF, "graph capture applies to effective-Hamiltonian plans");
     p.use_graphs = on != 0;
     if (!p.use_graphs) p.drop_graphs();
+  });
+}
+
+int qlg_plan_set_reorth(qlg_plan plan, int passes) {
+  if (!plan) return QLG_ERR_INVALID;
+  Plan &p = *plan->p;
+  return guarded(p.ctx, [&] {
+    QLG_CHECK(p.kind == PLAN_HEFF, "reorthogonalisation applies to effective-Hamiltonian plans");
+    QLG_CHECK(passes >= 0 && passes <= 2, "reorthogonalisation passes must be 0, 1 or 2");
+    if (p.reorth != passes) p.drop_graphs();
+    p.reorth = passes;
   });
 }
 
@@ -264,6 +275,7 @@ int qlg_plan_set_precision(qlg_plan plan, int precision) {
   return guarded(plan->p->ctx, [&] {
     QLG_CHECK(plan->p->kind == PLAN_HEFF, "precision applies to effective-Hamiltonian plans");
     check_precision(precision, plan->p->dtype);
+    if (plan->p->precision != precision) plan->p->drop_graphs();   // captured launches name the old variant
     plan->p->precision = precision;
   });
 }

@@ -88,7 +88,14 @@ struct GemmTile {
 
 struct Slots {
   double *p[NSLOTS];
+  // Operands held as bf16 piece planes (the split-BF16 modes): plane q of the
+  // slot's operand starts ps[slot] elements after plane q - 1.
+  int64_t ps[NSLOTS];
   const int64_t *crows;   // row-address table of tasks with c_rowtab set
+  // Device flag of a Krylov solve that runs ahead of the host: when it is set
+  // (the solve has stopped), every launch of the matvec returns at once.
+  // Null: always run.
+  const int *skip;
 };
 
 }  // namespace qlg

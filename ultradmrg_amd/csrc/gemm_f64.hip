@@ -55,6 +55,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_f64_kernel(const GemmTile *_
   const int wave = tid >> 6, lane = tid & 63;
   const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
   if ((int)blockIdx.x >= ntiles) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;  // padding slot of the XCD-balanced launch order
   const GemmTask tk = tasks[tl.task];

@@ -105,6 +105,7 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const GemmTile *__restric
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   if ((int)blockIdx.x >= ntiles) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
   const GemmTask tk = tasks[tl.task];
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_frag_kernel(const GemmTile *__r
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   if ((int)blockIdx.x >= ntiles) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
   const GemmTask tk = tasks[tl.task];
@@ -457,6 +459,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmTile *__restri
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   if ((int)blockIdx.x >= ntiles) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
   const GemmTask tk = tasks[tl.task];
@@ -649,6 +652,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_cplx_kernel(const GemmTile *__r
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   if ((int)blockIdx.x >= ntiles) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
   const GemmTask tk = tasks[tl.task];
@@ -803,6 +807,256 @@ __global__ __launch_bounds__(256) void gemm_bf16_cplx_kernel(const GemmTile *__r
     }
   }
   store_tile_cplx<FM, FN, TC>(tk, tl, slots, wm0, wn0, lane, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 on pre-split operands.  The A operand (and B when BPLANAR) is
+// held in memory as bf16 piece planes (h, m, l of each FP32 value, plane q at
+// ps[slot] elements after plane q - 1), laid out like the FP32 operand with a
+// leading dimension that is a multiple of 8 and zero padding past K in every
+// row.  A K slice is then staged by 16-byte loads of 8 consecutive k values
+// per plane, straight into the LDS piece planes: no split work.  A B operand
+// that is not planar (stage 1's B, the FP64 Krylov vector) is split when its
+// slice is staged, as in gemm_bf16_kernel.  The MFMA step is the same as
+// there.  Chunks of 8 k values that start at or past K read nothing (kOob);
+// a chunk that straddles K reads the row's zero padding.
+template <int BM, int BN, int NPC, typename TB, typename TC, bool BPLANAR>
+__global__ __launch_bounds__(256) void gemm_bf16p_kernel(const GemmTile *__restrict__ tiles,
+                                                         const GemmTask *__restrict__ tasks,
+                                                         const GemmSeg *__restrict__ segs, Slots slots, int ntiles) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int NPIECE = NPC == 3 ? 2 : 3;
+  constexpr int KC8 = BKB / 8;                                   // 8-value chunks per row of a slice
+  constexpr int CA = (BM * KC8 + 255) / 256, CB = (BN * KC8 + 255) / 256;   // chunks per thread (planar)
+  constexpr int KQ = BKB / 4;
+  constexpr int QB = BN * KQ / 256;                              // quads per thread (split B)
+  static_assert(BPLANAR || (QB >= 1 && BN * KQ % 256 == 0), "tile too small for 256 threads");
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  __shared__ bf16x8 Ap[2][NPIECE][BM][BKB / 8];
+  __shared__ bf16x8 Bp[2][NPIECE][BN][BKB / 8];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  if ((int)blockIdx.x >= ntiles) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
+  const GemmTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GemmTask tk = tasks[tl.task];
+  const int M = tk.M, N = tk.N;
+
+  flt4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+  u4 ra[NPIECE][CA];
+  u4 rbp[BPLANAR ? NPIECE : 1][BPLANAR ? CB : 1];
+  float rb[BPLANAR ? 1 : QB][4];
+
+  auto load = [&](const GemmSeg &sg, int k0) {
+    // A: planar, [M x K] rows with leading dimension lda
+    {
+      const __bf16 *a0 = reinterpret_cast<const __bf16 *>(slots.p[sg.a_slot]) + sg.a_off;
+      const int64_t aps = slots.ps[sg.a_slot];
+      const uint32_t lda = (uint32_t)sg.lda;
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p) {
+        const __amdgpu_buffer_rsrc_t ar = make_rsrc(a0 + p * aps);
+#pragma unroll
+        for (int i = 0; i < CA; ++i) {
+          const int q = tid + i * 256;
+          const int m = q / KC8, c = q % KC8;
+          const uint32_t gm = tl.m0 + m, gk = k0 + 8 * c;
+          const bool ok = q < BM * KC8 && (int)gm < M && (int)gk < sg.K;
+          ra[p][i] = __builtin_amdgcn_raw_buffer_load_b128(ar, ok ? (gm * lda + gk) * 2u : kOob, 0, 0);
+        }
+      }
+    }
+    if constexpr (BPLANAR) {
+      // B: planar, stored [N x K] (transposed operand) with leading dimension ldb
+      const __bf16 *b0 = reinterpret_cast<const __bf16 *>(slots.p[sg.b_slot]) + sg.b_off;
+      const int64_t bps = slots.ps[sg.b_slot];
+      const uint32_t ldb = (uint32_t)sg.ldb;
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p) {
+        const __amdgpu_buffer_rsrc_t br = make_rsrc(b0 + p * bps);
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+          const int q = tid + i * 256;
+          const int n = q / KC8, c = q % KC8;
+          const uint32_t gn = tl.n0 + n, gk = k0 + 8 * c;
+          const bool ok = q < BN * KC8 && (int)gn < N && (int)gk < sg.K;
+          rbp[p][i] = __builtin_amdgcn_raw_buffer_load_b128(br, ok ? (gn * ldb + gk) * 2u : kOob, 0, 0);
+        }
+      }
+    } else {
+      // B: TB values stored [K x N] (or [N x K] when transposed), split at staging
+      const __amdgpu_buffer_rsrc_t br = make_rsrc(reinterpret_cast<const TB *>(slots.p[sg.b_slot]) + sg.b_off);
+      const uint32_t ldb = (uint32_t)sg.ldb;
+      const bool tb = sg.flags & SEG_TRANS_B;
+#pragma unroll
+      for (int i = 0; i < QB; ++i) {
+        const int q = tid + i * 256;
+        const int n = tb ? q / KQ : q % BN, kq = tb ? q % KQ : q / BN;
+        const uint32_t gn = tl.n0 + n;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t gk = k0 + 4 * kq + j;
+          rb[i][j] = load_as_f32<TB>(br, ((int)gn < N && (int)gk < sg.K) ? (tb ? gn * ldb + gk : gk * ldb + gn) : kOob);
+        }
+      }
+    }
+  };
+  auto store = [&](int buf, uint8_t flags) {
+#pragma unroll
+    for (int p = 0; p < NPIECE; ++p)
+#pragma unroll
+      for (int i = 0; i < CA; ++i) {
+        const int q = tid + i * 256;
+        if (q < BM * KC8) {
+          const int m = q / KC8, c = q % KC8;
+          Ap[buf][p][m][swz(m, c)] = __builtin_bit_cast(bf16x8, ra[p][i]);
+        }
+      }
+    if constexpr (BPLANAR) {
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p)
+#pragma unroll
+        for (int i = 0; i < CB; ++i) {
+          const int q = tid + i * 256;
+          if (q < BN * KC8) {
+            const int n = q / KC8, c = q % KC8;
+            Bp[buf][p][n][swz(n, c)] = __builtin_bit_cast(bf16x8, rbp[p][i]);
+          }
+        }
+    } else {
+      const bool tb = flags & SEG_TRANS_B;
+#pragma unroll
+      for (int i = 0; i < QB; ++i) {
+        const int q = tid + i * 256;
+        const int n = tb ? q / KQ : q % BN, kq = tb ? q % KQ : q / BN;
+        bf16x4 pc[NPIECE];
+        split4<NPIECE>(rb[i], pc);
+#pragma unroll
+        for (int p = 0; p < NPIECE; ++p)
+          reinterpret_cast<bf16x4 *>(&Bp[buf][p][n][swz(n, kq >> 1)])[kq & 1] = pc[p];
+      }
+    }
+  };
+  auto step = [&](int buf) {
+    bf16x8 ap[FM][NPIECE], bp[FN][NPIECE];
+    const int c = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int row = wm0 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p) ap[i][p] = Ap[buf][p][row][swz(row, c)];
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int row = wn0 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int p = 0; p < NPIECE; ++p) bp[j][p] = Bp[buf][p][row][swz(row, c)];
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        flt4 cc = acc[i][j];
+        if constexpr (NPC == 6) {
+          cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][1], bp[j][1], cc, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][2], bp[j][0], cc, 0, 0, 0);
+          cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][2], cc, 0, 0, 0);
+        }
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][1], bp[j][0], cc, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][1], cc, 0, 0, 0);
+        cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i][0], bp[j][0], cc, 0, 0, 0);
+        acc[i][j] = cc;
+      }
+  };
+
+  const int s_end = tk.seg_begin + tk.seg_count;
+  int s = tk.seg_begin;
+  while (s < s_end && segs[s].K <= 0) ++s;
+  if (s < s_end) {
+    GemmSeg cur = segs[s];
+    int k0 = 0, buf = 0;
+    load(cur, 0);
+    store(0, cur.flags);
+    __syncthreads();
+    while (true) {
+      int ns = s, nk = k0 + BKB;
+      GemmSeg nxt = cur;
+      if (nk >= cur.K) {
+        ++ns;
+        nk = 0;
+        while (ns < s_end && segs[ns].K <= 0) ++ns;
+        if (ns < s_end) nxt = segs[ns];
+      }
+      const bool more = ns < s_end;
+      if (more) load(nxt, nk);
+      step(buf);
+      if (!more) break;
+      store(buf ^ 1, nxt.flags);
+      __syncthreads();
+      buf ^= 1;
+      s = ns;
+      k0 = nk;
+      cur = nxt;
+    }
+  }
+  store_tile<FM, FN, TC>(tk, tl, slots, wm0, wn0, lane, acc);
+}
+
+// FP64 -> NPIECE bf16 planes of the FP32 value (h, m, l), element for element:
+// the pre-split constant operands, made once per plan
+template <int NPIECE>
+__global__ __launch_bounds__(256) void split_planes_kernel(const double *__restrict__ src, __bf16 *__restrict__ dst,
+                                                           int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    float r = (float)src[i];
+#pragma unroll
+    for (int q = 0; q < NPIECE; ++q) {
+      const __bf16 h = (__bf16)r;
+      dst[q * n + i] = h;
+      r -= (float)h;
+    }
+  }
+}
+
+void launch_split_planes(const double *src, void *dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(split_planes_kernel<3>, dim3((unsigned)blocks), dim3(256), 0, st, src,
+                     static_cast<__bf16 *>(dst), n);
+  QLG_HIP(hipGetLastError());
+}
+
+// stage 1 (A planar, B = the FP64 vector split at staging, FP32 out) or
+// stage 3 (A and B planar, FP64 out through the row table)
+void launch_grouped_gemm_planar(int mode, int bm, int bn, bool stage3, const GemmTile *tiles, const GemmTask *tasks,
+                                const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st) {
+  if (ntiles <= 0) return;
+  dim3 grid(ntiles), block(256);
+#define QLG_BP(M_, N_, P_)                                                                                      \
+  if (bm == M_ && bn == N_) {                                                                                   \
+    if (stage3)                                                                                                 \
+      hipLaunchKernelGGL((gemm_bf16p_kernel<M_, N_, P_, float, double, true>), grid, block, 0, st, tiles, tasks, \
+                         segs, slots, ntiles);                                                                  \
+    else                                                                                                        \
+      hipLaunchKernelGGL((gemm_bf16p_kernel<M_, N_, P_, double, float, false>), grid, block, 0, st, tiles,      \
+                         tasks, segs, slots, ntiles);                                                           \
+    QLG_HIP(hipGetLastError());                                                                                 \
+    return;                                                                                                     \
+  }
+  if (mode == QLG_PREC_BF16X3) {
+    QLG_BP(32, 32, 3) QLG_BP(64, 32, 3) QLG_BP(32, 64, 3) QLG_BP(64, 64, 3) QLG_BP(128, 64, 3)
+    QLG_BP(64, 128, 3) QLG_BP(128, 128, 3)
+  } else if (mode == QLG_PREC_BF16X6) {
+    QLG_BP(32, 32, 6) QLG_BP(64, 32, 6) QLG_BP(32, 64, 6) QLG_BP(64, 64, 6) QLG_BP(128, 64, 6)
+    QLG_BP(64, 128, 6) QLG_BP(128, 128, 6)
+  }
+#undef QLG_BP
+  throw Error(QLG_ERR_UNSUPPORTED, "unsupported split-BF16 tile shape or mode");
 }
 
 // ---------------------------------------------------------------------------

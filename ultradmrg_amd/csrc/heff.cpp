@@ -267,19 +267,25 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   std::map<int, LStack> lst;
   std::vector<CopyDesc> lcopy;
   int64_t lint_elems = 0;
+  // Stacks, R panels and Y panels keep a leading dimension that is a multiple
+  // of 8 (zero padding past K in every row): the split-BF16 kernels stage
+  // their bf16 piece planes in aligned 16-byte chunks of 8 k values.
+  auto ld8 = [](int64_t k) { return (k + 7) / 8 * 8; };
+  int64_t lint_alg = 0, rint_alg = 0, y_alg = 0;   // elements without the padding
   for (auto &kv : L_by_ket) {
     const int a = kv.first;
     if (!fz.count(a)) continue;
-    const int64_t Da = L.ax[2].deg[a];
+    const int64_t Da = L.ax[2].deg[a], lda = ld8(Da);
     LStack s;
     s.base = lint_elems;
     for (int64_t lb : kv.second) {
       const int64_t r = L.ax[0].deg[L.blk[lb][0]] * L.ax[1].deg[L.blk[lb][1]];
       s.roff[lb] = s.rows;
-      lcopy.push_back(CopyDesc{L.off[lb], s.base + s.rows * Da, Da, Da, (int32_t)r, (int32_t)Da});
+      lcopy.push_back(CopyDesc{L.off[lb], s.base + s.rows * lda, Da, lda, (int32_t)r, (int32_t)Da});
       s.rows += r;
     }
-    lint_elems += s.rows * Da;
+    lint_elems += s.rows * lda;
+    lint_alg += s.rows * Da;
     lst[a] = s;
   }
   Stage &s1 = p.st[0], &s2 = p.st[1], &s3 = p.st[2];
@@ -294,7 +300,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       GemmSeg g{};
       g.a_slot = SLOT_L;
       g.a_off = s.base;
-      g.lda = Da;
+      g.lda = ld8(Da);
       g.b_slot = SLOT_IN;
       g.b_off = fz[a].base;
       g.ldb = N;
@@ -318,7 +324,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   pt.mark("stage1");
   // ---- R panels (bra-major) ------------------------------------------------------------
   struct RPanel {
-    int64_t base = 0, K = 0;
+    int64_t base = 0, K = 0, ld = 0;
     std::map<std::pair<int, int>, int64_t> col;   // (z sector, b sector) -> column offset
   };
   std::map<int, RPanel> rpn;
@@ -333,19 +339,21 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       P.K += R.ax[1].deg[R.blk[rb][1]] * R.ax[2].deg[R.blk[rb][2]];
     }
     const int64_t Dbp = R.ax[0].deg[bp];
+    P.ld = ld8(P.K);
     for (int64_t rb : kv.second) {
       const int64_t w = R.ax[1].deg[R.blk[rb][1]] * R.ax[2].deg[R.blk[rb][2]];
-      rcopy.push_back(CopyDesc{R.off[rb], P.base + P.col[{R.blk[rb][1], R.blk[rb][2]}], w, P.K, (int32_t)Dbp,
+      rcopy.push_back(CopyDesc{R.off[rb], P.base + P.col[{R.blk[rb][1], R.blk[rb][2]}], w, P.ld, (int32_t)Dbp,
                                (int32_t)w});
     }
-    rint_elems += Dbp * P.K;
+    rint_elems += Dbp * P.ld;
+    rint_alg += Dbp * P.K;
     rpn[bp] = P;
   }
 
   pt.mark("rpanels");
   // ---- Y panels (per bra sector) and stage 3 ---------------------------------------------
   struct YPanel {
-    int64_t base = 0, rows = 0, K = 0;
+    int64_t base = 0, rows = 0, K = 0, ld = 0;
   };
   std::map<int, YPanel> ypn;
   std::map<std::array<int64_t, 3>, int64_t> yrow;   // (vec block, comp0, comp1) -> first row
@@ -359,6 +367,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       YPanel Y;
       Y.base = ybase;
       Y.K = pit == rpn.end() ? 0 : pit->second.K;
+      Y.ld = ld8(Y.K);
       const int64_t Dbp = R.ax[0].deg[bp];
       const int32_t c_row0 = (int32_t)s3.crows.size();
       for (int64_t vb : kv.second) {
@@ -388,10 +397,10 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         GemmSeg g{};
         g.a_slot = SLOT_Y;
         g.a_off = Y.base;
-        g.lda = Y.K;
+        g.lda = Y.ld;
         g.b_slot = SLOT_R;
         g.b_off = pit->second.base;
-        g.ldb = Y.K;
+        g.ldb = pit->second.ld;
         g.K = (int32_t)Y.K;
         g.flags = SEG_TRANS_B;
         s3.segs.push_back(g);
@@ -399,7 +408,8 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         s3.flops += fl * double(Y.rows) * double(Dbp) * double(Y.K);
       }
       s3.tasks.push_back(t);
-      ybase += Y.rows * Y.K;
+      ybase += Y.rows * Y.ld;
+      y_alg += Y.rows * Y.K;
       ypn[bp] = Y;
     }
     p.y_elems = ybase;
@@ -577,8 +587,8 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       const YPanel &Y = ypn[bp];
       const int64_t row0 = yrow[{vb, o[1], o[3]}];
       RowDesc d;
-      d.off = Y.base + row0 * Y.K + rpn[bp].col[{o[4], c.b}] + (int64_t)o[5] * Db;
-      d.rs = Y.K;
+      d.off = Y.base + row0 * Y.ld + rpn[bp].col[{o[4], c.b}] + (int64_t)o[5] * Db;
+      d.rs = Y.ld;
       s2.rows.push_back(d);
     }
     // combined operator rows (outputs) x columns (inputs)
@@ -694,6 +704,10 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   p.Rint.ensure(std::max<int64_t>(rint_elems, 1) * eb);
   p.lint_elems = lint_elems;
   p.rint_elems = rint_elems;
+  // padding columns of the stacks and panels stay zero (the split-BF16
+  // kernels read them)
+  QLG_HIP(hipMemsetAsync(p.Lint.p, 0, std::max<int64_t>(lint_elems, 1) * eb, ctx->stream));
+  QLG_HIP(hipMemsetAsync(p.Rint.p, 0, std::max<int64_t>(rint_elems, 1) * eb, ctx->stream));
   pt.mark("alloc");
   // Y entries no channel writes stay zero for the lifetime of the plan
   QLG_HIP(hipMemsetAsync(p.Y.p, 0, std::max<int64_t>(p.y_elems, 1) * eb, ctx->stream));
@@ -717,9 +731,9 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   {
     // algorithmic HBM bytes per launch: operands read once, results written once
     const double e8 = 8.0 * ed;
-    p.st[0].algo_bytes = e8 * double(lint_elems + p.vec.size + p.x_elems);
-    p.st[1].algo_bytes = e8 * double(p.x_elems + p.c_elems + p.y_elems);
-    p.st[2].algo_bytes = e8 * double(p.y_elems + rint_elems + p.vec.size);
+    p.st[0].algo_bytes = e8 * double(lint_alg + p.vec.size + p.x_elems);
+    p.st[1].algo_bytes = e8 * double(p.x_elems + p.c_elems + y_alg);
+    p.st[2].algo_bytes = e8 * double(y_alg + rint_alg + p.vec.size);
   }
   if (const char *v = std::getenv("QLG_VERBOSE"); v && v[0] == '1') {
     const char *names[3] = {"L.psi", "MPO", ".R^T"};
@@ -784,12 +798,17 @@ Plan::~Plan() {
     if (e) (void)hipEventDestroy(e);
   drop_graphs();
   if (host_seg) (void)hipHostFree(host_seg);
+  if (host_lz) (void)hipHostFree(host_lz);
 }
 
 void Plan::drop_graphs() {
   for (auto &g : step_graphs)
     if (g) (void)hipGraphExecDestroy(g);
   step_graphs.clear();
+  for (auto &g : chunk_graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  chunk_graphs.clear();
+  chunk_key.clear();
 }
 
 void Plan::to_fused(const double *canon, double *fused) {
@@ -800,7 +819,27 @@ void Plan::to_canonical(const double *fused, double *canon) {
   launch_block_copy(dtype == QLG_C128, d_pack_out.as<CopyDesc>(), (int)pack_out.size(), fused, canon, ctx->stream);
 }
 
+bool Plan::planar() const {
+  return dtype == QLG_F64 && (precision == QLG_PREC_BF16X3 || precision == QLG_PREC_BF16X6);
+}
+
 void Plan::prepare_reduced() {
+  if (planar()) {
+    // real split-BF16: L stacks and R panels split once into three bf16 piece
+    // planes; Y kept as planes written by the MPO stage (padding and entries
+    // no channel writes stay zero); the channel operators in FP32
+    if (planar_ready) return;
+    Lp.ensure(std::max<int64_t>(3 * lint_elems, 1) * 2);
+    Rp.ensure(std::max<int64_t>(3 * rint_elems, 1) * 2);
+    Yp.ensure(std::max<int64_t>(3 * y_elems, 1) * 2);
+    C32.ensure(std::max<int64_t>(c_elems, 1) * sizeof(float));
+    QLG_HIP(hipMemsetAsync(Yp.p, 0, std::max<int64_t>(3 * y_elems, 1) * 2, ctx->stream));
+    launch_split_planes(Lint.as<double>(), Lp.p, lint_elems, ctx->stream);
+    launch_split_planes(Rint.as<double>(), Rp.p, rint_elems, ctx->stream);
+    launch_f64_to_f32(C.as<double>(), C32.as<float>(), c_elems, ctx->stream);
+    planar_ready = true;
+    return;
+  }
   if (reduced_ready) return;
   const int64_t ed = elem_doubles(dtype);   // complex data: (re, im) pairs of floats
   Lint32.ensure(std::max<int64_t>(lint_elems * ed, 1) * sizeof(float));
@@ -824,7 +863,16 @@ Slots Plan::slots(const double *in, double *out) {
   s.p[SLOT_X] = X.as<double>();
   s.p[SLOT_Y] = reduced ? reinterpret_cast<double *>(Y32.p) : Y.as<double>();
   s.p[SLOT_C] = reduced ? reinterpret_cast<double *>(C32.p) : C.as<double>();
+  if (planar()) {
+    s.p[SLOT_L] = static_cast<double *>(Lp.p);
+    s.p[SLOT_R] = static_cast<double *>(Rp.p);
+    s.p[SLOT_Y] = static_cast<double *>(Yp.p);
+    s.ps[SLOT_L] = lint_elems;
+    s.ps[SLOT_R] = rint_elems;
+    s.ps[SLOT_Y] = y_elems;
+  }
   s.crows = st[2].d_crows.as<int64_t>();
+  s.skip = skip_flag;
   return s;
 }
 
@@ -832,9 +880,16 @@ void Plan::launch_stage(int i, const Slots &s) {
   const bool reduced = precision != QLG_PREC_F64, cplx = dtype == QLG_C128;
   hipStream_t stream = ctx->stream;
   Stage &g = st[i];
+  const int out_planes = planar() ? (precision == QLG_PREC_BF16X6 ? 3 : 2) : 0;
   if (i == 1) {
-    launch_mpo_stage(g.bm, g.bn, cplx, reduced, g.d_tiles.as<GemmTile>(), g.d_chan.as<ChanTask>(),
-                     g.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], (int)g.tiles.size(), stream);
+    launch_mpo_stage(g.bm, g.bn, cplx, reduced, out_planes, y_elems, g.d_tiles.as<GemmTile>(),
+                     g.d_chan.as<ChanTask>(), g.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], s.skip,
+                     (int)g.tiles.size(), stream);
+    return;
+  }
+  if (out_planes) {
+    launch_grouped_gemm_planar(precision, g.bm, g.bn, i == 2, g.d_tiles.as<GemmTile>(), g.d_tasks.as<GemmTask>(),
+                               g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
     return;
   }
   // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1;
@@ -882,7 +937,7 @@ void Plan::autotune(const double *in, double *out) {
   }();
   // choices cached for this precision's shape signatures ({0, 0}: not tuned yet)
   std::pair<int, int> cached[3] = {{0, 0}, {0, 0}, {0, 0}};
-  bool unchanged = step_graphs.empty();
+  bool unchanged = step_graphs.empty() && chunk_graphs.empty();
   {
     std::lock_guard<std::mutex> lk(g_tile_mu);
     for (int i : {0, 2}) {

@@ -1,8 +1,18 @@
 // Krylov drivers on the GPU: Lanczos ground state and Lanczos exp(-i tau H)
 // psi.  The Krylov basis lives in one plan-owned device buffer; vector work
-// and the matvec run on the context stream; each iteration performs exactly
-// one device-to-host copy (the new diagonal element and the residual norm)
-// and solves the small FP64 tridiagonal problem on the host.
+// and the matvec run on the context stream.
+//
+// Single-chain solves run ahead of the host (lanczos_gs / lanczos_expmv):
+// every iteration ends with a device kernel that solves the FP64 tridiagonal
+// problem and applies the stop rule (krylov_dev.hip), and the host queues
+// iterations in chunks, looking at the device's stop flag once per chunk;
+// kernels queued past the stop return at once.  With graphs enabled on the
+// plan, each chunk is captured as a hipGraph on first use and replayed by
+// later solves.  Optional full reorthogonalisation (classical Gram-Schmidt
+// against the whole basis, once or twice per iteration) is off by default,
+// as in the reference.  The host-iteration forms (one device-to-host copy and
+// a host tridiagonal solve per iteration) remain for Krylov dimensions beyond
+// the device kernels' limits and for QLG_KRYLOV_HOST=1.
 //
 // Recurrences follow include/qlmps/algorithm/vmps/lanczos_vmps_solver_impl.h:114-248
 // and include/qlmps/algorithm/tdvp/lanczos_expmv_solver_impl.h:70-184:
@@ -14,10 +24,13 @@
 // discarded: the ground state only combines v_0..v_m.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "krylov.h"
+#include "krylov_dev.h"
 #include "tridiag.h"
 #include "vecops.h"
 
@@ -97,7 +110,7 @@ struct Workspace {
 
 }  // namespace
 
-void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos_result &res) {
+static void lanczos_gs_host(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos_result &res) {
   QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
   Ctx &c = *p.ctx;
   const int64_t maxit = prm.max_iterations;
@@ -371,7 +384,7 @@ void lanczos_gs_batch(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_l
   QLG_HIP(hipStreamSynchronize(st));
 }
 
-void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &prm, int64_t &iters) {
+static void lanczos_expmv_host(Plan &p, double *psi, double tau, const qlg_lanczos_params &prm, int64_t &iters) {
   QLG_CHECK(p.dtype == QLG_C128, "Lanczos expmv needs a complex (QLG_C128) plan");
   QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
   Ctx &c = *p.ctx;
@@ -439,6 +452,278 @@ void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &p
     }
     last = coef;
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// run-ahead single-chain solves
+// ---------------------------------------------------------------------------
+namespace {
+
+// iterations queued per look at the stop flag (QLG_LANCZOS_CHUNK, default 8)
+int chunk_iterations() {
+  const char *v = std::getenv("QLG_LANCZOS_CHUNK");
+  const int x = v ? std::atoi(v) : 8;
+  return x >= 1 ? x : 8;
+}
+
+// QLG_KRYLOV_HOST=1: the iteration-by-iteration host form
+bool host_iteration_forced() {
+  const char *v = std::getenv("QLG_KRYLOV_HOST");
+  return v && v[0] == '1';
+}
+
+// Device workspace of a run-ahead solve (in Plan::lz_ws):
+//   [LzCtl][ab: (a_k, b_k) pairs, maxit + 2][tmp: 8][coef_cur][coef_prev]
+//   (2 (maxit + 1) doubles each)[rc: Gram-Schmidt coefficients, 2 (maxit + 2)]
+//   [basis slot pointers, maxit + 1]
+struct RunAhead {
+  Plan &p;
+  Workspace w;
+  LzCtl *ctl;
+  double *ab, *tmp, *coef_cur, *coef_prev, *rc, *rpart = nullptr;
+  const double **ptrs;
+  int64_t maxit;
+  RunAhead(Plan &pl, int64_t mi) : p(pl), w(pl, mi), maxit(mi) {
+    const size_t n_ab = 2 * (maxit + 2), n_coef = 2 * (maxit + 1);
+    const size_t o_ab = 256, o_tmp = o_ab + n_ab * 8, o_cur = o_tmp + 64, o_prev = o_cur + n_coef * 8,
+                 o_rc = o_prev + n_coef * 8, o_ptr = (o_rc + n_ab * 8 + 255) / 256 * 256,
+                 total = o_ptr + (maxit + 1) * sizeof(double *);
+    p.lz_ws.ensure(total);
+    char *b = p.lz_ws.as<char>();
+    ctl = reinterpret_cast<LzCtl *>(b);
+    ab = reinterpret_cast<double *>(b + o_ab);
+    tmp = reinterpret_cast<double *>(b + o_tmp);
+    coef_cur = reinterpret_cast<double *>(b + o_cur);
+    coef_prev = reinterpret_cast<double *>(b + o_prev);
+    rc = reinterpret_cast<double *>(b + o_rc);
+    ptrs = reinterpret_cast<const double **>(b + o_ptr);
+    if (p.reorth > 0) {
+      p.reorth_ws.ensure((size_t)(maxit + 2) * kMaxPartials * 2 * sizeof(double));
+      rpart = p.reorth_ws.as<double>();
+    }
+    const size_t nh = 8 + n_ab;
+    if (p.host_lz_n < nh) {
+      if (p.host_lz) (void)hipHostFree(p.host_lz);
+      p.host_lz = nullptr;
+      p.host_lz_n = 0;
+      QLG_HIP(hipHostMalloc((void **)&p.host_lz, nh * sizeof(double), hipHostMallocDefault));
+      p.host_lz_n = nh;
+    }
+    std::vector<const double *> hp(maxit + 1);
+    for (int64_t i = 0; i <= maxit; ++i) hp[i] = w.slot(i);
+    Ctx &c = *p.ctx;
+    QLG_HIP(hipMemcpyAsync(ptrs, hp.data(), hp.size() * sizeof(double *), hipMemcpyHostToDevice, c.stream));
+    QLG_HIP(hipStreamSynchronize(c.stream));   // hp is a host temporary
+  }
+  double *A(int64_t m) { return ab + 2 * m; }
+  double *B(int64_t m) { return ab + 2 * m + 1; }
+  LzCtl fetch_ctl() {
+    Ctx &c = *p.ctx;
+    QLG_HIP(hipMemcpyAsync(p.host_lz, ctl, sizeof(LzCtl), hipMemcpyDeviceToHost, c.stream));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+    LzCtl h;
+    std::memcpy(&h, p.host_lz, sizeof(LzCtl));
+    return h;
+  }
+  void fetch_ab(int64_t n, std::vector<double> &a, std::vector<double> &b) {
+    Ctx &c = *p.ctx;
+    QLG_HIP(hipMemcpyAsync(p.host_lz, ab, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+    a.assign(n, 0.0);
+    b.assign(n, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      a[i] = p.host_lz[2 * i];
+      b[i] = p.host_lz[2 * i + 1];
+    }
+  }
+  // residual of iteration m in slot m+1 is final: reorthogonalise it and
+  // replace b_m by its norm
+  void reorth(int64_t m, const int *skip) {
+    for (int pass = 0; pass < p.reorth; ++pass)
+      lz_reorth(p.ctx->stream, p.dtype, w.n, ptrs, (int)(m + 1), w.slot(m + 1), rpart, rc, B(m), skip);
+  }
+  // queue iterations [m0, m1) with `body`, as a captured graph when the plan
+  // uses graphs (one graph per chunk index, replayed by later solves with the
+  // same key)
+  template <class F>
+  void run_chunk(int64_t m0, int64_t m1, int64_t chunk, const std::string &key, F body) {
+    Ctx &c = *p.ctx;
+    if (!p.use_graphs) {
+      for (int64_t m = m0; m < m1; ++m) body(m);
+      return;
+    }
+    if (p.chunk_key != key) {
+      for (auto &g : p.chunk_graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+      p.chunk_graphs.clear();
+      p.chunk_key = key;
+    }
+    if ((int64_t)p.chunk_graphs.size() <= chunk) p.chunk_graphs.resize(chunk + 1, nullptr);
+    if (!p.chunk_graphs[chunk]) {
+      hipGraph_t g = nullptr;
+      QLG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      p.capturing = true;
+      try {
+        for (int64_t m = m0; m < m1; ++m) body(m);
+      } catch (...) {
+        p.capturing = false;
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      p.capturing = false;
+      QLG_HIP(hipStreamEndCapture(c.stream, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      QLG_HIP(e);
+      p.chunk_graphs[chunk] = ex;
+    }
+    QLG_HIP(hipGraphLaunch(p.chunk_graphs[chunk], c.stream));
+  }
+};
+
+// the plan's matvecs read the solve's stop flag while it is queued
+struct SkipGuard {
+  Plan &p;
+  SkipGuard(Plan &pl, const int *flag) : p(pl) { p.skip_flag = flag; }
+  ~SkipGuard() { p.skip_flag = nullptr; }
+};
+
+std::string graph_key(const RunAhead &r, const char *what, double err, double tau) {
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%s|%lld|%p|%p|%d|%d|%.17g|%.17g|%lld|%d", what, (long long)r.maxit,
+                (const void *)r.w.slot0, (const void *)r.ctl, r.p.precision, r.p.reorth, err, tau,
+                (long long)r.w.n, chunk_iterations());
+  return buf;
+}
+
+}  // namespace
+
+void lanczos_gs(Plan &p, double *psi, const qlg_lanczos_params &prm, qlg_lanczos_result &res) {
+  QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
+  if (host_iteration_forced() || prm.max_iterations > 2047) {
+    QLG_CHECK(p.reorth == 0, "reorthogonalisation needs the device-resident solver (max_iterations <= 2047)");
+    lanczos_gs_host(p, psi, prm, res);
+    return;
+  }
+  Ctx &c = *p.ctx;
+  hipStream_t st = c.stream;
+  const int64_t maxit = prm.max_iterations;
+  RunAhead r(p, maxit);
+  Workspace &w = r.w;
+  const int64_t dim = w.n;
+  double h[2];
+  // v0 = psi / ||psi||   (QuasiNormalize, lanczos_vmps_solver_impl.h:141)
+  p.to_fused(psi, w.slot(0));
+  vec_nrm2(c, p.dtype, w.n, w.slot(0), w.dev_tmp);
+  w.fetch(w.dev_tmp, h, 1);
+  QLG_CHECK(h[0] > 0.0, "initial state has zero norm");
+  vec_scal(c, p.dtype, w.n, 1.0 / h[0], 0.0, w.slot(0));
+  lz_reset(st, r.ctl);
+  // iteration 0: a_0, the first residual and its norm
+  p.matvec_fused(w.slot(0), w.slot(1));   // also settles the tile shapes before any capture
+  {
+    SkipGuard sg(p, &r.ctl->stop);
+    vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), r.A(0), &r.ctl->stop);
+    vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), r.A(0), nullptr, nullptr, r.B(0), &r.ctl->stop);
+    r.reorth(0, &r.ctl->stop);
+    lz_gs_check(st, r.ab, r.ctl, 0, dim, maxit, prm.error);
+    auto body = [&](int64_t m) {
+      const int *skip = &r.ctl->stop;
+      vec_scal_inv_dev(c, p.dtype, w.n, r.B(m - 1), w.slot(m), skip);
+      p.matvec_fused(w.slot(m), w.slot(m + 1));
+      vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), r.A(m), skip);
+      vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), r.A(m), w.slot(m - 1), r.B(m - 1), r.B(m),
+                           skip);
+      r.reorth(m, skip);
+      lz_gs_check(st, r.ab, r.ctl, (int)m, dim, maxit, prm.error);
+    };
+    const int64_t K = chunk_iterations();
+    const std::string key = graph_key(r, "gs", prm.error, 0.0);
+    for (int64_t m0 = 1, chunk = 0;; m0 += K, ++chunk) {
+      const int64_t m1 = std::min(m0 + K, maxit);
+      if (m0 < m1) r.run_chunk(m0, m1, chunk, key, body);
+      if (r.fetch_ctl().stop) break;
+      QLG_CHECK(m1 < maxit, "device Lanczos: no stop by max_iterations");
+    }
+  }
+  const LzCtl cs = r.fetch_ctl();
+  const int64_t iters = cs.iters;
+  QLG_CHECK(iters >= 1 && iters <= maxit, "device Lanczos: invalid stop");
+  std::vector<double> a, b, vec(iters);
+  r.fetch_ab(iters, a, b);
+  double e;
+  if (!tridiag_gs(iters, a.data(), b.data(), true, &e, vec.data()))
+    throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+  w.combine(iters, vec.data(), false, psi);
+  res.iters = iters;
+  res.gs_eng = p.precision == QLG_PREC_F64 ? e : w.rayleigh_f64();
+}
+
+void lanczos_expmv(Plan &p, double *psi, double tau, const qlg_lanczos_params &prm, int64_t &iters) {
+  QLG_CHECK(p.dtype == QLG_C128, "Lanczos expmv needs a complex (QLG_C128) plan");
+  QLG_CHECK(prm.max_iterations >= 2, "max_iterations must be >= 2");
+  if (host_iteration_forced() || prm.max_iterations > 64) {
+    QLG_CHECK(p.reorth == 0, "reorthogonalisation needs the device-resident solver (max_iterations <= 64)");
+    lanczos_expmv_host(p, psi, tau, prm, iters);
+    return;
+  }
+  Ctx &c = *p.ctx;
+  hipStream_t st = c.stream;
+  const int64_t maxit = prm.max_iterations;
+  RunAhead r(p, maxit);
+  Workspace &w = r.w;
+  const int64_t dim = w.n;
+  double h[2];
+  // Normalize keeps the initial norm (lanczos_expmv_solver_impl.h:93)
+  p.to_fused(psi, w.slot(0));
+  vec_nrm2(c, p.dtype, w.n, w.slot(0), w.dev_tmp);
+  w.fetch(w.dev_tmp, h, 1);
+  const double initial_norm = h[0];
+  QLG_CHECK(initial_norm > 0.0, "initial state has zero norm");
+  vec_scal(c, p.dtype, w.n, 1.0 / initial_norm, 0.0, w.slot(0));
+  lz_reset(st, r.ctl);
+  p.matvec_fused(w.slot(0), w.slot(1));
+  {
+    SkipGuard sg(p, &r.ctl->stop);
+    // a_m = Re <H v_m | v_m>: the complex dot writes (re, im) over (a_m, b_m);
+    // the residual reads the real part and then overwrites b_m with its norm
+    vec_dot(c, p.dtype, w.n, w.slot(1), w.slot(0), r.A(0), &r.ctl->stop);
+    vec_lanczos_residual(c, p.dtype, w.n, w.slot(1), w.slot(0), r.A(0), nullptr, nullptr, r.B(0), &r.ctl->stop);
+    r.reorth(0, &r.ctl->stop);
+    lz_expmv_check(st, r.ab, r.ctl, r.coef_cur, r.coef_prev, nullptr, 0, dim, maxit, prm.error, tau);
+    auto body = [&](int64_t m) {
+      const int *skip = &r.ctl->stop;
+      vec_scal_inv_dev(c, p.dtype, w.n, r.B(m - 1), w.slot(m), skip);
+      p.matvec_fused(w.slot(m), w.slot(m + 1));
+      vec_dot(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), r.A(m), skip);
+      vec_lanczos_residual(c, p.dtype, w.n, w.slot(m + 1), w.slot(m), r.A(m), w.slot(m - 1), r.B(m - 1), r.B(m),
+                           skip);
+      r.reorth(m, skip);
+      lz_expmv_check(st, r.ab, r.ctl, r.coef_cur, r.coef_prev, nullptr, (int)m, dim, maxit, prm.error, tau);
+    };
+    const int64_t K = chunk_iterations();
+    const std::string key = graph_key(r, "expmv", prm.error, tau);
+    for (int64_t m0 = 1, chunk = 0;; m0 += K, ++chunk) {
+      const int64_t m1 = std::min(m0 + K, maxit);
+      if (m0 < m1) r.run_chunk(m0, m1, chunk, key, body);
+      if (r.fetch_ctl().stop) break;
+      QLG_CHECK(m1 < maxit, "device expmv: no stop by max_iterations");
+    }
+  }
+  const LzCtl cs = r.fetch_ctl();
+  if (cs.iters < 0) throw Error(QLG_ERR_NUMERIC, "tridiagonal eigen-solver did not converge");
+  QLG_CHECK(cs.iters >= 1 && cs.iters <= maxit, "device expmv: invalid stop");
+  iters = cs.iters;
+  // psi <- ||psi_0|| sum_i c_i v_i with the device's coefficients
+  p.vtmp_out.ensure((size_t)w.nd * sizeof(double));
+  double *y = p.vtmp_out.as<double>();
+  vec_lincomb(c, p.dtype, w.n, (int)iters, r.coef_cur, r.ptrs, true, 0.0, 0.0, false, y);
+  vec_scal(c, p.dtype, w.n, initial_norm, 0.0, y);
+  p.to_canonical(y, psi);
+  QLG_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace qlg

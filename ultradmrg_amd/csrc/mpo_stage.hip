@@ -52,12 +52,15 @@ struct Mfma<float> {    // v_mfma_f32_16x16x4: D lane l, reg r -> row 4 (l / 16)
 
 // T: element type of C, X and Y (double, or float for the reduced-precision
 // modes); CPLX: complex data of that element type, stored (re, im).
-template <int BM, int BN, bool CPLX, typename T>
+// OUTP > 0 (real FP32 only): Y is written as OUTP bf16 piece planes of each
+// FP32 result x, h = bf16(x), m = bf16(x - h), l = bf16(x - h - m), plane q
+// at yps elements after plane q - 1 (the operand of the pre-split stage 3).
+template <int BM, int BN, bool CPLX, typename T, int OUTP = 0>
 __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restrict__ tiles,
                                                         const ChanTask *__restrict__ tasks,
                                                         const RowDesc *__restrict__ rows, const T *__restrict__ C,
                                                         const T *__restrict__ X, T *__restrict__ Y,
-                                                        int ntiles) {
+                                                        const int *__restrict__ skip, int64_t yps, int ntiles) {
   using MF = Mfma<T>;
   typedef typename MF::acc_t acc_t;
   constexpr int WN = BN / 4;                // columns per wave
@@ -72,6 +75,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   if ((int)blockIdx.x >= ntiles) return;
+  if (skip && *skip) return;   // the Krylov solve has stopped
   const GemmTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
   const ChanTask tk = tasks[tl.task];
@@ -186,7 +190,15 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
         if (tl.m0 + lr < M) {
           const RowDesc rd = rout[lr];
           const int64_t ix = rd.off + (int64_t)apj[j] * rd.rs + bj[j];
-          if constexpr (CPLX && sizeof(T) == 8)
+          if constexpr (OUTP > 0) {
+            float x = acc[0][i][j][r];
+#pragma unroll
+            for (int q = 0; q < OUTP; ++q) {
+              const __bf16 h = (__bf16)x;
+              reinterpret_cast<__bf16 *>(Y)[q * yps + ix] = h;
+              x -= (float)h;
+            }
+          } else if constexpr (CPLX && sizeof(T) == 8)
             reinterpret_cast<double2 *>(Y)[ix] = double2{acc[0][i][j][r], acc[1][i][j][r]};
           else if constexpr (CPLX)
             reinterpret_cast<float2 *>(Y)[ix] = float2{acc[0][i][j][r], acc[1][i][j][r]};
@@ -248,8 +260,9 @@ void launch_trans_copy(bool cplx, const TransDesc *d, int n, const double *src, 
   QLG_HIP(hipGetLastError());
 }
 
-void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles, const ChanTask *tasks,
-                      const RowDesc *rows, const void *C, const void *X, void *Y, int ntiles, hipStream_t st) {
+void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const GemmTile *tiles,
+                      const ChanTask *tasks, const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip,
+                      int ntiles, hipStream_t st) {
   if (ntiles <= 0) return;
   dim3 grid(ntiles), block(256);
   const double *Cd = static_cast<const double *>(C), *Xd = static_cast<const double *>(X);
@@ -258,18 +271,26 @@ void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles
   float *Yf = static_cast<float *>(Y);
 #define QLG_MPO(M_, N_)                                                                                       \
   if (bm == M_ && bn == N_) {                                                                                 \
-    if (f32 && cplx)                                                                                          \
+    if (out_planes == 3 && f32 && !cplx)                                                                      \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float, 3>), grid, block, 0, st, tiles, tasks, rows,  \
+                         Cf, Xf, Yf, skip, yps, ntiles);                                                      \
+    else if (out_planes == 2 && f32 && !cplx)                                                                 \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float, 2>), grid, block, 0, st, tiles, tasks, rows,  \
+                         Cf, Xf, Yf, skip, yps, ntiles);                                                      \
+    else if (out_planes != 0)                                                                                 \
+      throw Error(QLG_ERR_UNSUPPORTED, "MPO stage: piece-plane output is for real FP32 data");               \
+    else if (f32 && cplx)                                                                                     \
       hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, float>), grid, block, 0, st, tiles, tasks, rows, Cf,  \
-                         Xf, Yf, ntiles);                                                                     \
+                         Xf, Yf, skip, 0, ntiles);                                                                     \
     else if (f32)                                                                                             \
       hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float>), grid, block, 0, st, tiles, tasks, rows, Cf, \
-                         Xf, Yf, ntiles);                                                                     \
+                         Xf, Yf, skip, 0, ntiles);                                                                     \
     else if (cplx)                                                                                            \
       hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, double>), grid, block, 0, st, tiles, tasks, rows, Cd, \
-                         Xd, Yd, ntiles);                                                                     \
+                         Xd, Yd, skip, 0, ntiles);                                                                     \
     else                                                                                                      \
       hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, double>), grid, block, 0, st, tiles, tasks, rows,   \
-                         Cd, Xd, Yd, ntiles);                                                                 \
+                         Cd, Xd, Yd, skip, 0, ntiles);                                                                 \
     QLG_HIP(hipGetLastError());                                                                               \
     return;                                                                                                   \
   }

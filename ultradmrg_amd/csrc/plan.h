@@ -86,11 +86,26 @@ struct Plan {
   // reads Y entries that no channel writes, which must stay zero.
   DevBuf Lint32, Rint32, C32, Y32;
   bool reduced_ready = false;
+  // real split-BF16 modes: L stacks, R panels and Y as three bf16 piece
+  // planes each (h, m, l of the FP32 value; the 3-term form reads h and m)
+  DevBuf Lp, Rp, Yp;
+  bool planar_ready = false;
+  bool planar() const;
   void prepare_reduced();
   DevBuf vtmp_in, vtmp_out;     // staging for qlg_matvec on canonical vectors
   // batch plans: member m's vector occupies [batch_off[m], batch_off[m+1])
   // of the batch vector (canonical and fused layouts alike)
   std::vector<int64_t> batch_off;
+  // single-chain Krylov solves run ahead of the host (krylov.cpp): device
+  // control block and scalars, the skip flag their matvecs read (Slots::skip),
+  // optional full reorthogonalisation, and hipGraphs of iteration chunks
+  int reorth = 0;                  // classical Gram-Schmidt passes per iteration (0: off, the reference)
+  const int *skip_flag = nullptr;  // set while a run-ahead solve is being queued
+  DevBuf lz_ws, reorth_ws;
+  double *host_lz = nullptr;       // pinned copies of the control block and the (a, b) scalars
+  size_t host_lz_n = 0;
+  std::vector<hipGraphExec_t> chunk_graphs;
+  std::string chunk_key;
   // hipGraphs of the batched Lanczos steps (index m), captured on first use
   bool use_graphs = false;
   std::vector<hipGraphExec_t> step_graphs;
@@ -122,8 +137,14 @@ void check_pair(const Axis &a, const Axis &b, const char *what);
 std::vector<int64_t> sector_offsets(const Axis &a);
 std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed);
 void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
-void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, const GemmTile *tiles, const ChanTask *tasks,
-                      const RowDesc *rows, const void *C, const void *X, void *Y, int ntiles, hipStream_t st);
+void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const GemmTile *tiles,
+                      const ChanTask *tasks, const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip,
+                      int ntiles, hipStream_t st);
+// split-BF16 on pre-split operands (gemm_mixed.hip): stage 1 (A planes, B the
+// FP64 vector) or stage 3 (A and B planes, FP64 output); FP64 -> 3 bf16 planes
+void launch_grouped_gemm_planar(int mode, int bm, int bn, bool stage3, const GemmTile *tiles, const GemmTask *tasks,
+                                const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st);
+void launch_split_planes(const double *src, void *dst, int64_t n, hipStream_t st);
 void launch_grouped_gemm_mixed(int mode, int bm, int bn, bool b_f64, bool c_f64, bool cplx, const GemmTile *tiles,
                                const GemmTask *tasks, const GemmSeg *segs, const Slots &slots, int ntiles,
                                hipStream_t st);

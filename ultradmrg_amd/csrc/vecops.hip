@@ -41,7 +41,8 @@ __device__ inline void block_sum(double (&v)[NC], double (*sh)[VT / 64]) {
 template <bool CPLX>
 __global__ __launch_bounds__(VT) void dot_partial_kernel(int64_t n, const double *__restrict__ x,
                                                          const double *__restrict__ y,
-                                                         double *__restrict__ partials) {
+                                                         double *__restrict__ partials, const int *__restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int NC = CPLX ? 2 : 1;
   __shared__ double sh[NC][VT / 64];
   double acc[NC];
@@ -66,7 +67,9 @@ __global__ __launch_bounds__(VT) void dot_partial_kernel(int64_t n, const double
 
 template <int NC>
 __global__ __launch_bounds__(VT) void reduce_partials_kernel(const double *__restrict__ partials, int g,
-                                                             double *__restrict__ out, int sqrt_first) {
+                                                             double *__restrict__ out, int sqrt_first,
+                                                             const int *__restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sh[NC][VT / 64];
   double acc[NC];
 #pragma unroll
@@ -80,6 +83,15 @@ __global__ __launch_bounds__(VT) void reduce_partials_kernel(const double *__res
     for (int c = 0; c < NC; ++c) out[c] = acc[c];
     if (sqrt_first) out[0] = sqrt(acc[0]);
   }
+}
+
+// x <- x / *nrm (real scale; complex data as 2n doubles), skipped once the solve has stopped
+__global__ __launch_bounds__(VT) void scal_inv_dev_kernel(int64_t nd, const double *__restrict__ nrm,
+                                                          double *__restrict__ x, const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  const double alpha = 1.0 / *nrm;
+  const int64_t stride = (int64_t)gridDim.x * VT;
+  for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < nd; i += stride) x[i] *= alpha;
 }
 
 __global__ __launch_bounds__(VT) void scal_kernel(int64_t nd, double alpha, double *__restrict__ x) {
@@ -157,7 +169,9 @@ __global__ __launch_bounds__(VT) void lanczos_residual_kernel(int64_t nd, double
                                                               const double *__restrict__ a_ptr,
                                                               const double *__restrict__ v0,
                                                               const double *__restrict__ b_ptr,
-                                                              double *__restrict__ partials) {
+                                                              double *__restrict__ partials,
+                                                              const int *__restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sh[1][VT / 64];
   double acc[1] = {0.0};
   const double a = *a_ptr;
@@ -181,26 +195,39 @@ static int grid_for(int64_t n) {
   return (int)g;
 }
 
-void vec_dot(Ctx &c, int dtype, int64_t n, const double *x, const double *y, double *dev_out) {
+void dot_into(hipStream_t st, int dtype, int64_t n, const double *x, const double *y, double *part,
+              double *dev_out, const int *skip) {
   const bool cplx = dtype == QLG_C128;
   const int g = grid_for(n);
-  double *part = c.partials.as<double>();
   if (cplx) {
-    hipLaunchKernelGGL(dot_partial_kernel<true>, dim3(g), dim3(VT), 0, c.stream, n, x, y, part);
-    hipLaunchKernelGGL(reduce_partials_kernel<2>, dim3(1), dim3(VT), 0, c.stream, part, g, dev_out, 0);
+    hipLaunchKernelGGL(dot_partial_kernel<true>, dim3(g), dim3(VT), 0, st, n, x, y, part, skip);
+    hipLaunchKernelGGL(reduce_partials_kernel<2>, dim3(1), dim3(VT), 0, st, part, g, dev_out, 0, skip);
   } else {
-    hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(g), dim3(VT), 0, c.stream, n, x, y, part);
-    hipLaunchKernelGGL(reduce_partials_kernel<1>, dim3(1), dim3(VT), 0, c.stream, part, g, dev_out, 0);
+    hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(g), dim3(VT), 0, st, n, x, y, part, skip);
+    hipLaunchKernelGGL(reduce_partials_kernel<1>, dim3(1), dim3(VT), 0, st, part, g, dev_out, 0, skip);
   }
   QLG_HIP(hipGetLastError());
 }
 
-void vec_nrm2(Ctx &c, int dtype, int64_t n, const double *x, double *dev_out) {
+void nrm2_into(hipStream_t st, int dtype, int64_t n, const double *x, double *part, double *dev_out) {
   const int64_t nd = n * elem_doubles(dtype);
   const int g = grid_for(nd);
-  double *part = c.partials.as<double>();
-  hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(g), dim3(VT), 0, c.stream, nd, x, x, part);
-  hipLaunchKernelGGL(reduce_partials_kernel<1>, dim3(1), dim3(VT), 0, c.stream, part, g, dev_out, 1);
+  hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(g), dim3(VT), 0, st, nd, x, x, part, nullptr);
+  hipLaunchKernelGGL(reduce_partials_kernel<1>, dim3(1), dim3(VT), 0, st, part, g, dev_out, 1, nullptr);
+  QLG_HIP(hipGetLastError());
+}
+
+void vec_dot(Ctx &c, int dtype, int64_t n, const double *x, const double *y, double *dev_out, const int *skip) {
+  dot_into(c.stream, dtype, n, x, y, c.partials.as<double>(), dev_out, skip);
+}
+
+void vec_nrm2(Ctx &c, int dtype, int64_t n, const double *x, double *dev_out) {
+  nrm2_into(c.stream, dtype, n, x, c.partials.as<double>(), dev_out);
+}
+
+void vec_scal_inv_dev(Ctx &c, int dtype, int64_t n, const double *nrm, double *x, const int *skip) {
+  const int64_t nd = n * elem_doubles(dtype);
+  hipLaunchKernelGGL(scal_inv_dev_kernel, dim3(grid_for(nd)), dim3(VT), 0, c.stream, nd, nrm, x, skip);
   QLG_HIP(hipGetLastError());
 }
 
@@ -238,12 +265,13 @@ void vec_lincomb(Ctx &c, int dtype, int64_t n, int k, const double *dev_coefs, c
 }
 
 void vec_lanczos_residual(Ctx &c, int dtype, int64_t n, double *w, const double *v1, const double *a_dev,
-                          const double *v0, const double *b_dev, double *dev_norm_out) {
+                          const double *v0, const double *b_dev, double *dev_norm_out, const int *skip) {
   const int64_t nd = n * elem_doubles(dtype);
   const int g = grid_for(nd);
   double *part = c.partials.as<double>();
-  hipLaunchKernelGGL(lanczos_residual_kernel, dim3(g), dim3(VT), 0, c.stream, nd, w, v1, a_dev, v0, b_dev, part);
-  hipLaunchKernelGGL(reduce_partials_kernel<1>, dim3(1), dim3(VT), 0, c.stream, part, g, dev_norm_out, 1);
+  hipLaunchKernelGGL(lanczos_residual_kernel, dim3(g), dim3(VT), 0, c.stream, nd, w, v1, a_dev, v0, b_dev, part,
+                     skip);
+  hipLaunchKernelGGL(reduce_partials_kernel<1>, dim3(1), dim3(VT), 0, c.stream, part, g, dev_norm_out, 1, skip);
   QLG_HIP(hipGetLastError());
 }
 

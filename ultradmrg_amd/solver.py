@@ -164,6 +164,16 @@ class EffectiveHamiltonian:
     def set_precision(self, precision: str):
         self.ctx.check(self.lib.qlg_plan_set_precision(self.h, PRECISIONS[precision]))
 
+    def set_reorth(self, passes: int):
+        """Full reorthogonalisation of each Lanczos residual against the whole
+        basis (classical Gram-Schmidt, ``passes`` = 0, 1 or 2 per iteration)."""
+        self.ctx.check(self.lib.qlg_plan_set_reorth(self.h, int(passes)))
+
+    def use_graphs(self, on: bool = True):
+        """Capture the Krylov solvers' iteration chunks as hipGraphs on first use
+        and replay them in later solves on this plan."""
+        self.ctx.check(self.lib.qlg_plan_use_graphs(self.h, int(on)))
+
     def enable_timing(self, on=True):
         self.ctx.check(self.lib.qlg_plan_enable_timing(self.h, int(on)))
 
