@@ -222,6 +222,21 @@ int qlg_axpy(qlg_ctx ctx, int dtype, int64_t n, const double *alpha,
 int qlg_linear_combine(qlg_ctx ctx, int dtype, int64_t n, int64_t k,
                        const double *coefs, const void *const *bases,
                        const double *beta, void *y);
+/* qlg_linear_combine that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code between hipGraph captures, tight serving loops, harnesses that
+ * forbid hipMalloc).  qlg_linear_combine stages its coefficient and pointer
+ * tables in device buffers it allocates per call; this entry point stages
+ * them in a fixed array in the library's device image instead (24 KiB per
+ * GPU, held until the process ends; its address is looked up when a context
+ * is created on the device).  Limit: k <= 1024 basis vectors; a larger k is
+ * refused with QLG_ERR_INVALID.  Same arguments, results and rounding as
+ * qlg_linear_combine; returns after the work is complete.  All calls on one
+ * GPU share the array: calls that overlap in time on one GPU (from several
+ * threads or contexts) race and must be serialised by the caller.        */
+int qlg_linear_combine_noalloc(qlg_ctx ctx, int dtype, int64_t n, int64_t k,
+                               const double *coefs, const void *const *bases,
+                               const double *beta, void *y);
 
 /* ---- Krylov solvers ---------------------------------------------------------
  * qlg_lanczos_gs: LanczosSolver, lanczos_vmps_solver_impl.h:114-248 (plain

@@ -31,7 +31,7 @@ EXPORTS = [
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
     "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
     "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
-    "qlg_linear_combine", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_eigh_batched", "qlg_plan_batch", "qlg_plan_batch_offsets",
+    "qlg_linear_combine", "qlg_linear_combine_noalloc", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_eigh_batched", "qlg_plan_batch", "qlg_plan_batch_offsets",
     "qlg_plan_use_graphs", "qlg_plan_set_reorth", "qlg_lanczos_gs_batch", "qlg_tridiag_gs",
     "qlg_tridiag_expme1",
 ]
@@ -117,6 +117,7 @@ def lib():
         "qlg_scal": (C.c_int, [vp, C.c_int, i64, dp, vp]),
         "qlg_axpy": (C.c_int, [vp, C.c_int, i64, dp, vp, vp]),
         "qlg_linear_combine": (C.c_int, [vp, C.c_int, i64, i64, dp, C.POINTER(vp), dp, vp]),
+        "qlg_linear_combine_noalloc": (C.c_int, [vp, C.c_int, i64, i64, dp, C.POINTER(vp), dp, vp]),
         "qlg_lanczos_gs": (C.c_int, [vp, vp, C.POINTER(LanczosParams), C.POINTER(LanczosResult)]),
         "qlg_lanczos_expmv": (C.c_int, [vp, vp, C.c_double, C.POINTER(LanczosParams), C.POINTER(i64)]),
         "qlg_eigh_batched": (C.c_int, [vp, C.c_int, i64, C.POINTER(EighGroup)]),

@@ -7,6 +7,7 @@
 
 #include "eigh.h"
 #include "krylov.h"
+#include "noalloc.h"
 #include "plan.h"
 #include "tridiag.h"
 #include "vecops.h"
@@ -71,6 +72,7 @@ int qlg_ctx_create(int device, void *stream, qlg_ctx *out) {
     }
     h->c.partials.ensure(2 * kMaxPartials * sizeof(double));
     h->c.scalars.ensure(64 * sizeof(double));
+    noalloc_init(device);   // address of the fixed staging array on this device
     QLG_HIP(hipHostMalloc((void **)&h->c.host_scalars, 64 * sizeof(double), hipHostMallocDefault));
     const char *pool = std::getenv("QLG_POOL");
     if (!(pool && pool[0] == '0')) h->c.pool = std::make_shared<DevPool>(device);
@@ -439,6 +441,32 @@ int qlg_linear_combine(qlg_ctx ctx, int dtype, int64_t n, int64_t k, const doubl
     const double br = beta[0], bi = cplx ? beta[1] : 0.0;
     vec_lincomb(c, dtype, n, (int)k, cb.as<double>(), pb.as<const double *>(), cplx, br, bi,
                 br != 0.0 || bi != 0.0, static_cast<double *>(y));
+    QLG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int qlg_linear_combine_noalloc(qlg_ctx ctx, int dtype, int64_t n, int64_t k, const double *coefs,
+                               const void *const *bases, const double *beta, void *y) {
+  if (!ctx || !coefs || !bases || !beta || n < 0 || k < 0) return QLG_ERR_INVALID;
+  return guarded(&ctx->c, [&] {
+    Ctx &c = ctx->c;
+    QLG_CHECK(dtype == QLG_F64 || dtype == QLG_C128, "dtype must be QLG_F64 or QLG_C128");
+    QLG_CHECK(k <= kNoAllocMaxBases, "qlg_linear_combine_noalloc: k = " + std::to_string(k) +
+                                          " exceeds the staging limit of " + std::to_string(kNoAllocMaxBases) +
+                                          " basis vectors");
+    const bool cplx = dtype == QLG_C128;
+    // coefficients then basis pointers, in the library's fixed staging array
+    double *stage = noalloc_scratch(c.device);
+    double *dcoef = stage;
+    const double **dptr = reinterpret_cast<const double **>(stage + 2 * kNoAllocMaxBases);
+    if (k > 0) {
+      QLG_HIP(hipMemcpyAsync(dcoef, coefs, k * (cplx ? 2 : 1) * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      QLG_HIP(hipMemcpyAsync(dptr, bases, k * sizeof(void *), hipMemcpyHostToDevice, c.stream));
+    }
+    const double br = beta[0], bi = cplx ? beta[1] : 0.0;
+    vec_lincomb(c, dtype, n, (int)k, dcoef, dptr, cplx, br, bi, br != 0.0 || bi != 0.0, static_cast<double *>(y));
+    // the host tables are read by the copies above, and the staging array is
+    // shared by every call on this device: return only when the work is done
     QLG_HIP(hipStreamSynchronize(c.stream));
   });
 }
