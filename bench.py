@@ -52,6 +52,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "eff-H matvec TFLOP/s + sweep sec at D=2000: split-BF16 vs FP64, |dE|<1e-8"
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 STAGES = ["stage1 L.psi", "stage2 MPO", "stage3 .R^T"]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_traffic_hubbard_d2000.txt")
@@ -82,10 +83,62 @@ def recorded_traffic(D, precision, stage):
     return float(v) if v is not None else None
 
 
-def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 1000, 2000, 2000, 2000), D=2000,
-                             nsweeps=2):
-    """Seconds per D=2000 sweep in FP64, in the split mode and in the ``alt``
-    modes, each from the same ramped state."""
+def captured_matvec(drv, site, modes, reps=20):
+    """The reference profiler's measurement (profiler/single_processor_lanczos_performance_test.cc:76-119)
+    on tensors captured from a running sweep: the two-site effective Hamiltonian
+    at sites (site, site + 1) of ``drv``'s current state, timed in each
+    precision mode, with per-stage kernel times and roofline fractions."""
+    import torch
+    from ultradmrg_amd.solver import EffectiveHamiltonian
+    from ultradmrg_amd.tensor import contract
+    n = len(drv.mps)
+    for i in range(site):              # centre to (site, site + 1): left-canonicalise 0 .. site - 1
+        drv.update("r", i)
+    psi = contract(drv.mps[site], drv.mps[site + 1], 1)
+    H = EffectiveHamiltonian([drv.lenvs[site], drv.mpo[site], drv.mpo[site + 1], drv.renvs[n - site - 2]], psi)
+    x = H.adopt(psi)
+    y = H.new_vector(zero=False)
+    algo, _ = H.flops()
+    s_algo, s_bytes = H.stage_work()
+    bond = drv.mps[site].axes[2]
+    out = {"site": [site, site + 1], "bond_dim": sum(bond.degs), "bond_sectors": bond.nsect,
+           "largest_sector": max(bond.degs), "psi_elems": H.size, "algo_gflop_per_matvec": algo / 1e9}
+    for mode in modes:
+        H.set_precision(mode)
+        ms = time_matvec(H, x, y, reps, 3)
+        H.enable_timing(True)
+        acc = [0.0, 0.0, 0.0]
+        for _ in range(10):
+            H.matvec_into(x.data, y.data, sync=True)
+            acc = [p + q for p, q in zip(acc, H.stage_ms())]
+        H.enable_timing(False)
+        st = [v / 10 for v in acc]
+        rec = {"ms_per_matvec": ms, "tflops": algo / (ms * 1e-3) / 1e12, "stage_ms": st,
+               "stage_tflops": [f / (t * 1e-3) / 1e12 if t > 0 else None for f, t in zip(s_algo, st)],
+               "mpo_stage_gbs": s_bytes[1] / (st[1] * 1e-3) / 1e9 if st[1] > 0 else None}
+        if mode == "fp64":
+            rec["frac_of_fp64_peak"] = rec["tflops"] / FP64_PEAK_TFLOPS
+        else:
+            mult = {"bf16x6": 6, "bf16x3": 3}.get(mode, 1)
+            rec["issued_frac_of_bf16_peak_stage3"] = s_algo[2] * mult / (st[2] * 1e-3) / 1e12 / BF16_PEAK_TFLOPS
+        out[mode] = rec
+    for mode in modes:
+        if mode != "fp64":
+            out[mode]["speedup_vs_fp64"] = out["fp64"]["ms_per_matvec"] / out[mode]["ms_per_matvec"]
+    H.close()
+    return out
+
+
+def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 1000), D=2000, max_converge=12,
+                             conv_tol=1e-8, nsweeps=2):
+    """Config 3: the 4x16 Hubbard cylinder (t=1, U=8, 56 electrons, Sz=0, the
+    reference benchmark's initial product state, benchmark/ultradmrg_hubbard2d.cpp:130-248;
+    Lanczos 1e-9 / 100, truncation 1e-8) ramped through ``ramp`` (one sweep
+    each), then swept at D=2000 until two consecutive sweep energies agree to
+    ``conv_tol`` relative (at most ``max_converge`` sweeps, the reference's 12 per
+    D).  From that converged state ``nsweeps`` sweeps are timed in FP64, in the
+    split mode and in the ``alt`` modes; the matvec at the chain centre is then
+    timed on the captured converged tensors (``matvec_real``)."""
     import torch
     from ultradmrg_amd import mpo as M
     from ultradmrg_amd.solver import LanczosParams
@@ -107,11 +160,18 @@ def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 
     for Dr in ramp:
         drv.p = SweepParams(1, Dr, Dr, 1e-8, lp, verbose=False)
         drv.sweep()
+    drv.p = SweepParams(1, D, D, 1e-8, lp, verbose=False)
+    conv = [drv.sweep()]
+    while len(conv) < max_converge:
+        conv.append(drv.sweep())
+        if abs(conv[-1] - conv[-2]) <= conv_tol * abs(conv[-1]):
+            break
     ramp_s = time.perf_counter() - t0
-    ramp_e = drv.stats.log[-1]["e0"]
     saved = (list(drv.mps), list(drv.lenvs), list(drv.renvs))   # updates replace tensors, never mutate them
     out = {"model": f"Hubbard {lx}x4 cylinder, U=8, t=1, {n - n // per_hole} electrons, Sz=0",
-           "ramp": list(ramp), "ramp_s": ramp_s, "ramp_energy": ramp_e, "D": D, "timed_sweeps": nsweeps}
+           "ramp": list(ramp), "D": D, "converge_sweeps": len(conv), "converge_energies": conv,
+           "converged_rel_change": abs(conv[-1] - conv[-2]) / abs(conv[-1]) if len(conv) > 1 else None,
+           "ramp_and_converge_s": ramp_s, "timed_sweeps": nsweeps}
     for prec in ("fp64", split) + tuple(alt):
         drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
         drv.p = SweepParams(1, D, D, 1e-8, lp, precision=prec, verbose=False)
@@ -132,6 +192,10 @@ def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 
     out["rel_dE"] = abs(es - e64) / abs(e64)
     out["alt"] = {m: {"speedup": sum(out["fp64"]["sweep_s"]) / sum(out[m]["sweep_s"]),
                       "rel_dE": abs(out[m]["energy"] - e64) / abs(e64)} for m in alt}
+    # the matvec on the converged state's centre tensors
+    drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
+    drv.p = SweepParams(1, D, D, 1e-8, lp, verbose=False)
+    out["matvec_real"] = captured_matvec(drv, n // 2 - 1, ("fp64", split) + tuple(alt))
     return out
 
 
@@ -346,6 +410,7 @@ def main():
     ap.add_argument("--no-chains", action="store_true")
     ap.add_argument("--no-tdvp", action="store_true")
     ap.add_argument("--no-chain", action="store_true", help="skip the L=100 Heisenberg ramp")
+    ap.add_argument("--tdvp-steps", type=int, default=3, help="TDVP steps timed per form (config 5 runs 40)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this library does not shard the matvec across GPUs; "
@@ -440,9 +505,10 @@ def main():
     if not a.no_chains:
         out["batching"] = chains_comparison()
     if not a.no_tdvp:
-        out["tdvp"] = tdvp_measurement(split=a.split)
+        out["tdvp"] = tdvp_measurement(split=a.split, steps=a.tdvp_steps)
     if not a.no_sweep:
         out["sweep"] = hubbard_sweep_comparison(split=a.split, alt=alt)
+        out["matvec_real"] = out["sweep"].pop("matvec_real")
     if not a.no_cpu:
         from tests.convert import from_device
         out["cpu_baseline"] = cpu_baseline([from_device(t) for t in d_eff], from_device(psi_host))

@@ -294,11 +294,42 @@ int qlg_plan_use_graphs(qlg_plan plan, int on);
 int qlg_lanczos_gs_batch(qlg_plan plan, void *psi_inout, const qlg_lanczos_params *params,
                          qlg_lanczos_result *res);
 
-/* ---- batched Hermitian eigen-decomposition (block SVD) -------------------------
- * The qlten SVD of the two-site state (two_site_update_finite_vmps_impl.h:302-307,
- * dmrg_impl.h:249-254, two_site_update_finite_tdvp_impl.h:339-344) is computed
- * from the eigen-decompositions of its sector Gram matrices; this entry point
- * solves them.  Group k holds batch matrices of order n, row-major and
+/* ---- block SVD with truncation -------------------------------------------------
+ * The qlten SVD of the reference's two-site updates,
+ *   SVD(t, ldims, div, trunc_err, Dmin, Dmax, u, s, vt, &actual_trunc_err, &D)
+ * (two_site_update_finite_vmps_impl.h:302-307, dmrg_impl.h:249-254,
+ * two_site_update_finite_tdvp_impl.h:339-344).  t's first ldims axes are the
+ * rows, the others the columns; ldiv (qn_len values) is the divergence the
+ * left factor u keeps.  The new bond carries one sector per charge
+ * q = ldiv - flow(left axes) with kept singular values, in ascending charge
+ * order.  Truncation (this library's rule; qlten's is not part of the
+ * reference): pool the singular values of all sectors and keep the smallest D
+ * whose discarded weight sum(s^2)/total <= trunc_err, clamped to
+ * [Dmin, Dmax]; actual_trunc_err is that discarded weight.  No value is
+ * rounded to zero: kept singular values below ~1e-6 of their sector's largest
+ * are resolved again on their own scale; vectors of exactly zero values are
+ * completed to an orthonormal set.  Results live in the handle:
+ *   u  [left axes..., bond (OUT)], same dtype as t;
+ *   s  [bond (IN), bond (OUT)], diagonal blocks, always QLG_F64;
+ *   vt [bond (IN), right axes...], same dtype as t;
+ * each in the canonical layout (blocks in lexicographic order of their sector
+ * coordinates, offsets = prefix sums).  qlg_svd_result gives D, the
+ * truncation error and the bond's sector count; qlg_svd_bond its sectors;
+ * qlg_svd_tensor (which: 0 u, 1 s, 2 vt) a tensor's block list and device
+ * data, valid until qlg_svd_destroy.  Array arguments may be NULL to query
+ * counts first.                                                              */
+typedef struct qlg_svd_s *qlg_svd;
+int qlg_block_svd(qlg_ctx ctx, const qlg_bsten *t, int32_t ldims, const int32_t *ldiv, double trunc_err,
+                  int64_t Dmin, int64_t Dmax, qlg_svd *out);
+int qlg_svd_result(qlg_svd r, double *trunc_err, int64_t *D, int64_t *bond_nsect);
+int qlg_svd_bond(qlg_svd r, int32_t *qn /* nsect * qn_len */, int64_t *deg /* nsect */);
+int qlg_svd_tensor(qlg_svd r, int which, int64_t *nblocks, int64_t *size, int32_t *coords, int64_t *offsets,
+                   const void **data);
+int qlg_svd_destroy(qlg_svd r);
+
+/* ---- batched Hermitian eigen-decomposition ---------------------------------------
+ * The solver qlg_block_svd runs on the Gram matrices of its sectors, exported
+ * for callers with their own batches.  Group k holds batch matrices of order n, row-major and
  * Hermitian, at a (dtype QLG_F64 or QLG_C128); on return row j of each matrix
  * holds eigenvector j (complex-conjugated for QLG_C128) and w[0..n) its
  * eigenvalues in ascending order.  Divide and conquer; groups run

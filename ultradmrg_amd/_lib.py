@@ -31,6 +31,7 @@ EXPORTS = [
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
     "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
     "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
+    "qlg_block_svd", "qlg_svd_result", "qlg_svd_bond", "qlg_svd_tensor", "qlg_svd_destroy",
     "qlg_linear_combine", "qlg_linear_combine_noalloc", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_eigh_batched", "qlg_plan_batch", "qlg_plan_batch_offsets",
     "qlg_plan_use_graphs", "qlg_plan_set_reorth", "qlg_lanczos_gs_batch", "qlg_tridiag_gs",
     "qlg_tridiag_expme1",
@@ -118,6 +119,12 @@ def lib():
         "qlg_axpy": (C.c_int, [vp, C.c_int, i64, dp, vp, vp]),
         "qlg_linear_combine": (C.c_int, [vp, C.c_int, i64, i64, dp, C.POINTER(vp), dp, vp]),
         "qlg_linear_combine_noalloc": (C.c_int, [vp, C.c_int, i64, i64, dp, C.POINTER(vp), dp, vp]),
+        "qlg_block_svd": (C.c_int, [vp, vp, C.c_int32, C.POINTER(C.c_int32), C.c_double, i64, i64, C.POINTER(vp)]),
+        "qlg_svd_result": (C.c_int, [vp, dp, C.POINTER(i64), C.POINTER(i64)]),
+        "qlg_svd_bond": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(i64)]),
+        "qlg_svd_tensor": (C.c_int, [vp, C.c_int, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_int32),
+                                     C.POINTER(i64), C.POINTER(vp)]),
+        "qlg_svd_destroy": (C.c_int, [vp]),
         "qlg_lanczos_gs": (C.c_int, [vp, vp, C.POINTER(LanczosParams), C.POINTER(LanczosResult)]),
         "qlg_lanczos_expmv": (C.c_int, [vp, vp, C.c_double, C.POINTER(LanczosParams), C.POINTER(i64)]),
         "qlg_eigh_batched": (C.c_int, [vp, C.c_int, i64, C.POINTER(EighGroup)]),

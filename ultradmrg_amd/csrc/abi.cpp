@@ -9,6 +9,7 @@
 #include "krylov.h"
 #include "noalloc.h"
 #include "plan.h"
+#include "svd.h"
 #include "tridiag.h"
 #include "vecops.h"
 
@@ -19,6 +20,9 @@ struct qlg_ctx_s {
 };
 struct qlg_plan_s {
   std::unique_ptr<Plan> p;
+};
+struct qlg_svd_s {
+  std::unique_ptr<SvdResult> r;
 };
 
 namespace {
@@ -487,6 +491,62 @@ int qlg_lanczos_expmv(qlg_plan plan, void *psi_inout, double tau, const qlg_lanc
     QLG_CHECK(p.kind == PLAN_HEFF, "qlg_lanczos_expmv needs an effective-Hamiltonian plan");
     lanczos_expmv(p, static_cast<double *>(psi_inout), tau, *params, *iters);
   });
+}
+
+int qlg_block_svd(qlg_ctx ctx, const qlg_bsten *t, int32_t ldims, const int32_t *ldiv, double trunc_err,
+                  int64_t Dmin, int64_t Dmax, qlg_svd *out) {
+  if (!ctx || !t || !ldiv || !out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    auto h = std::make_unique<qlg_svd_s>();
+    h->r.reset(block_svd(ctx->c, t, ldims, ldiv, trunc_err, Dmin, Dmax));
+    *out = h.release();
+  });
+}
+
+int qlg_svd_result(qlg_svd r, double *trunc_err, int64_t *D, int64_t *bond_nsect) {
+  if (!r) return QLG_ERR_INVALID;
+  if (trunc_err) *trunc_err = r->r->trunc_err;
+  if (D) *D = r->r->D;
+  if (bond_nsect) *bond_nsect = (int64_t)r->r->bond_deg.size();
+  return QLG_OK;
+}
+
+int qlg_svd_bond(qlg_svd r, int32_t *qn, int64_t *deg) {
+  if (!r) return QLG_ERR_INVALID;
+  const SvdResult &s = *r->r;
+  for (size_t i = 0; i < s.bond_deg.size(); ++i) {
+    if (qn)
+      for (int k = 0; k < s.qn_len; ++k) qn[i * s.qn_len + k] = s.bond_qn[i][k];
+    if (deg) deg[i] = s.bond_deg[i];
+  }
+  return QLG_OK;
+}
+
+int qlg_svd_tensor(qlg_svd r, int which, int64_t *nblocks, int64_t *size, int32_t *coords, int64_t *offsets,
+                   const void **data) {
+  if (!r || which < 0 || which > 2) return QLG_ERR_INVALID;
+  const SvdResult &s = *r->r;
+  const int64_t nb = (int64_t)s.offsets[which].size();
+  if (nblocks) *nblocks = nb;
+  if (size) *size = s.size[which];
+  if (coords)
+    for (size_t i = 0; i < s.coords[which].size(); ++i) coords[i] = s.coords[which][i];
+  if (offsets)
+    for (int64_t i = 0; i < nb; ++i) offsets[i] = s.offsets[which][i];
+  if (data) *data = s.data[which].p;
+  return QLG_OK;
+}
+
+int qlg_svd_destroy(qlg_svd r) {
+  if (!r) return QLG_OK;
+  Ctx *c = r->r->ctx;
+  if (c) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  delete r;
+  return QLG_OK;
 }
 
 int qlg_eigh_batched(qlg_ctx ctx, int dtype, int64_t ngroups, const qlg_eigh_group *groups) {

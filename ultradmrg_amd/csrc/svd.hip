@@ -1,0 +1,434 @@
+// Device kernels of the block SVD (svd.cpp): gathering sector matrices into
+// padded batches, per-matrix norms and scaling, the Gram matrices' padding,
+// the complex batches' Householder rotation, row norms, Newton-Schulz
+// updates, completion of zero rows, and the assembly of u, s and vt blocks.
+// Complex data is interleaved (re, im); every kernel takes the element size
+// in doubles (ed = 1 real, 2 complex).
+#include <hip/hip_runtime.h>
+
+#include "svd.h"
+
+namespace qlg {
+
+namespace {
+constexpr int VT = 256;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double block_sum(double v) {
+  __shared__ double sh[VT / 64];
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < VT / 64; ++w) s += sh[w];
+  return s;
+}
+__device__ inline double block_max(double v) {
+  __shared__ double sh[VT / 64];
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < VT / 64; ++w) s = fmax(s, sh[w]);
+  return s;
+}
+
+// one workgroup per block: element (r, c) of the block's [rows x cols] row-major
+// data goes to row rr = r0 + r, column cc = c0 + c of its sector matrix, stored
+// transposed (short side first) when tr is set
+__global__ __launch_bounds__(VT) void gather_kernel(const SvdBlk *__restrict__ tab, const double *__restrict__ src,
+                                                    double *__restrict__ dst, int ed) {
+  const SvdBlk b = tab[blockIdx.x];
+  const int64_t tot = b.rows * b.cols;
+  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+    const int64_t r = e / b.cols, c = e % b.cols;
+    const int64_t rr = b.r0 + r, cc = b.c0 + c;
+    const int64_t pos = b.base + (b.tr ? cc * b.ld + rr : rr * b.ld + cc);
+    for (int q = 0; q < ed; ++q) dst[pos * ed + q] = src[(b.src + e) * ed + q];
+  }
+}
+
+// scale[i] = Frobenius norm of matrix i (rows x cols within leading dimension ld), then
+// the matrix is divided by it (norm 0: left as is, scale 1)
+__global__ __launch_bounds__(VT) void normalise_kernel(const SvdMat *__restrict__ mats, double *__restrict__ x,
+                                                       double *__restrict__ scale, int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  double s = 0.0;
+  const int64_t tot = m.rows * m.cols;
+  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+    const int64_t p = m.off + (e / m.cols) * m.ld + e % m.cols;
+    for (int q = 0; q < ed; ++q) s += x[p * ed + q] * x[p * ed + q];
+  }
+  const double nrm = sqrt(block_sum(s));
+  const double inv = nrm > 0.0 ? 1.0 / nrm : 1.0;
+  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+    const int64_t p = m.off + (e / m.cols) * m.ld + e % m.cols;
+    for (int q = 0; q < ed; ++q) x[p * ed + q] *= inv;
+  }
+  if (threadIdx.x == 0) scale[blockIdx.x] = nrm > 0.0 ? nrm : 1.0;
+}
+
+// Gram matrix padding: diagonal entries j >= n of the b x b matrix get
+// -1 - j / b (distinct values below the spectrum)
+__global__ __launch_bounds__(VT) void pad_diag_kernel(const SvdMat *__restrict__ mats, double *__restrict__ g,
+                                                      int ed) {
+  const SvdMat m = mats[blockIdx.x];   // rows = b, cols = n (valid order)
+  for (int64_t j = m.cols + threadIdx.x; j < m.rows; j += VT) {
+    const int64_t p = m.off + j * m.ld + j;
+    g[p * ed] = -1.0 - (double)j / (double)m.rows;
+    if (ed == 2) g[p * ed + 1] = 0.0;
+  }
+}
+
+// complex Hermitian G (b x b) <- H G H with H = I - 2 u u^H (u unit):
+// w = G u, G <- G - 2 u w^H - 2 w u^H + 4 (u^H w) u u^H
+__global__ __launch_bounds__(VT) void reflect_kernel(const SvdMat *__restrict__ mats, double2 *__restrict__ g,
+                                                     const double2 *__restrict__ u) {
+  extern __shared__ double2 w[];
+  const SvdMat m = mats[blockIdx.x];
+  const int64_t b = m.rows;
+  const double2 *ub = u + m.aux;   // this order's reflector vector
+  for (int64_t i = threadIdx.x; i < b; i += VT) {
+    double2 s{0.0, 0.0};
+    for (int64_t j = 0; j < b; ++j) {
+      const double2 a = g[m.off + i * m.ld + j], v = ub[j];
+      s.x += a.x * v.x - a.y * v.y;
+      s.y += a.x * v.y + a.y * v.x;
+    }
+    w[i] = s;
+  }
+  __syncthreads();
+  double sc = 0.0;   // Re(u^H w)
+  for (int64_t i = threadIdx.x; i < b; i += VT) sc += ub[i].x * w[i].x + ub[i].y * w[i].y;
+  sc = block_sum(sc);
+  for (int64_t e = threadIdx.x; e < b * b; e += VT) {
+    const int64_t i = e / b, j = e % b;
+    const double2 ui = ub[i], uj = ub[j], wi = w[i], wj = w[j];
+    // u_i conj(w_j), w_i conj(u_j), u_i conj(u_j)
+    const double2 uw{ui.x * wj.x + ui.y * wj.y, ui.y * wj.x - ui.x * wj.y};
+    const double2 wu{wi.x * uj.x + wi.y * uj.y, wi.y * uj.x - wi.x * uj.y};
+    const double2 uu{ui.x * uj.x + ui.y * uj.y, ui.y * uj.x - ui.x * uj.y};
+    double2 &a = g[m.off + i * m.ld + j];
+    a.x += -2.0 * uw.x - 2.0 * wu.x + 4.0 * sc * uu.x;
+    a.y += -2.0 * uw.y - 2.0 * wu.y + 4.0 * sc * uu.y;
+  }
+}
+
+// rows r of the b x b matrix E^H (row j = e_j^H): r <- r H = r - 2 (r u) u^H
+__global__ __launch_bounds__(VT) void reflect_rows_kernel(const SvdMat *__restrict__ mats, double2 *__restrict__ e,
+                                                          const double2 *__restrict__ u) {
+  const SvdMat m = mats[blockIdx.x];
+  const int64_t b = m.rows;
+  const double2 *ub = u + m.aux;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t r = wave; r < b; r += VT / 64) {
+    double2 *row = e + m.off + r * m.ld;
+    double tr = 0.0, ti = 0.0;
+    for (int64_t j = lane; j < b; j += 64) {
+      tr += row[j].x * ub[j].x - row[j].y * ub[j].y;
+      ti += row[j].x * ub[j].y + row[j].y * ub[j].x;
+    }
+    tr = wave_sum(tr);
+    ti = wave_sum(ti);
+    for (int64_t j = lane; j < b; j += 64) {
+      // - 2 t conj(u_j)
+      row[j].x -= 2.0 * (tr * ub[j].x + ti * ub[j].y);
+      row[j].y -= 2.0 * (ti * ub[j].x - tr * ub[j].y);
+    }
+  }
+}
+
+// norms of rows: one wave per row of each matrix; out[m.aux + r]
+__global__ __launch_bounds__(VT) void row_norms_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ x,
+                                                       double *__restrict__ out, int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t r = wave; r < m.rows; r += VT / 64) {
+    double s = 0.0;
+    for (int64_t c = lane; c < m.cols; c += 64) {
+      const int64_t p = m.off + r * m.ld + c;
+      for (int q = 0; q < ed; ++q) s += x[p * ed + q] * x[p * ed + q];
+    }
+    s = wave_sum(s);
+    if (lane == 0) out[m.aux + r] = sqrt(s);
+  }
+}
+
+// gather rows with a permutation and a scale: dst row i (of k) = src row perm[i] * f[i]
+__global__ __launch_bounds__(VT) void gather_rows_kernel(const SvdRows *__restrict__ jobs, const int32_t *__restrict__ perm,
+                                                         const double *__restrict__ f, const double *__restrict__ src,
+                                                         double *__restrict__ dst, int ed) {
+  const SvdRows j = jobs[blockIdx.x];
+  const int64_t tot = j.k * j.cols;
+  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+    const int64_t i = e / j.cols, c = e % j.cols;
+    const int64_t sr = perm[j.perm + i];
+    const double fi = f ? f[j.perm + i] : 1.0;
+    for (int q = 0; q < ed; ++q)
+      dst[(j.dst + i * j.dld + c) * ed + q] = src[(j.src + sr * j.sld + c) * ed + q] * fi;
+  }
+}
+
+// Newton-Schulz deviation: max over (i, j) of |A_ij - mask_i delta_ij| for each k x k matrix
+__global__ __launch_bounds__(VT) void ns_dev_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ a,
+                                                    const double *__restrict__ mask, double *__restrict__ dev, int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  double mx = 0.0;
+  const int64_t k = m.rows;
+  for (int64_t e = threadIdx.x; e < k * k; e += VT) {
+    const int64_t i = e / k, j = e % k;
+    const int64_t p = m.off + i * m.ld + j;
+    double re = a[p * ed], im = ed == 2 ? a[p * ed + 1] : 0.0;
+    if (i == j) re -= mask[m.aux + i];
+    mx = fmax(mx, sqrt(re * re + im * im));
+  }
+  mx = block_max(mx);
+  if (threadIdx.x == 0) dev[blockIdx.x] = mx;
+}
+
+// p <- 1.5 p - 0.5 t over each listed k x cols matrix (contiguous rows)
+__global__ __launch_bounds__(VT) void ns_update_kernel(const SvdMat *__restrict__ mats, double *__restrict__ p,
+                                                       const double *__restrict__ t, int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  const int64_t tot = m.rows * m.cols * ed;
+  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+    const int64_t i = m.off * ed + e;
+    p[i] = 1.5 * p[i] - 0.5 * t[m.aux * ed + e];
+  }
+}
+
+// Rows of a k x cols matrix whose mask is 0 (zero singular values) are replaced,
+// one after another, by unit vectors orthogonal to every other row: a start
+// vector (unit vector on a cycling coordinate) is orthogonalised twice by
+// classical Gram-Schmidt against all rows with mask 1 or already completed,
+// then normalised.  One workgroup per matrix.
+__global__ __launch_bounds__(VT) void complete_rows_kernel(const SvdMat *__restrict__ mats, double *__restrict__ p,
+                                                           double *__restrict__ mask, int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  __shared__ double cf[2];
+  const int64_t k = m.rows, n = m.cols;
+  int64_t start = 0;
+  for (int64_t z = 0; z < k; ++z) {
+    if (mask[m.aux + z] != 0.0) continue;
+    double *rz = p + (m.off + z * m.ld) * ed;
+    // try start coordinates until one leaves a nonzero residual
+    for (int64_t attempt = 0; attempt < n; ++attempt, ++start) {
+      for (int64_t c = threadIdx.x; c < n * ed; c += VT) rz[c] = 0.0;
+      __syncthreads();
+      if (threadIdx.x == 0) rz[(start % n) * ed] = 1.0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass)
+        for (int64_t i = 0; i < k; ++i) {
+          if (i == z || mask[m.aux + i] == 0.0) continue;
+          const double *ri = p + (m.off + i * m.ld) * ed;
+          double sr = 0.0, si = 0.0;   // <r_i, r_z> = sum conj(r_i) r_z
+          for (int64_t c = threadIdx.x; c < n; c += VT) {
+            if (ed == 2) {
+              sr += ri[2 * c] * rz[2 * c] + ri[2 * c + 1] * rz[2 * c + 1];
+              si += ri[2 * c] * rz[2 * c + 1] - ri[2 * c + 1] * rz[2 * c];
+            } else {
+              sr += ri[c] * rz[c];
+            }
+          }
+          sr = block_sum(sr);
+          si = block_sum(si);
+          for (int64_t c = threadIdx.x; c < n; c += VT) {
+            if (ed == 2) {
+              rz[2 * c] -= sr * ri[2 * c] - si * ri[2 * c + 1];
+              rz[2 * c + 1] -= sr * ri[2 * c + 1] + si * ri[2 * c];
+            } else {
+              rz[c] -= sr * ri[c];
+            }
+          }
+          __syncthreads();
+        }
+      double s = 0.0;
+      for (int64_t c = threadIdx.x; c < n * ed; c += VT) s += rz[c] * rz[c];
+      const double nrm = sqrt(block_sum(s));
+      if (nrm > 1e-6) {
+        for (int64_t c = threadIdx.x; c < n * ed; c += VT) rz[c] /= nrm;
+        ++start;
+        break;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) mask[m.aux + z] = 1.0;
+    __syncthreads();
+  }
+}
+
+// One-sided (Hestenes) Jacobi on the n rows of R (length m) of one sector per
+// workgroup, with every rotation applied to the same two rows of E^H (length
+// b) as well, so X = E R is kept exactly: rows i, j with alpha = |r_i|^2,
+// beta = |r_j|^2, gamma = <r_i, r_j> = |gamma| e^{i phi} are replaced by
+//   r_i <- c r_i - s e^{i phi} r_j,  r_j <- s r_i + c e^{i phi} r_j
+// with t = s / c the smaller root of t^2 + 2 zeta t - 1 = 0,
+// zeta = (beta - alpha) / (2 |gamma|), which makes them orthogonal.  Pairs run
+// in round-robin order (all disjoint pairs of a round in parallel, one wave
+// each); sweeps repeat until no pair has |gamma| > 1e-15 sqrt(alpha beta)
+// (at most 40).  Afterwards the row norms are the singular values.
+__global__ __launch_bounds__(1024) void jacobi_rows_kernel(const SvdJac *__restrict__ jobs, double *__restrict__ r,
+                                                           double *__restrict__ e, int ed) {
+  const SvdJac j = jobs[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int64_t n = j.n, np = n + (n & 1);   // players, with a dummy when n is odd
+  __shared__ int rotated;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (threadIdx.x == 0) rotated = 0;
+    __syncthreads();
+    for (int64_t rd = 0; rd < np - 1; ++rd) {
+      for (int64_t pi = wave; pi < np / 2; pi += nw) {
+        int64_t a, b;
+        if (pi == 0) {
+          a = np - 1;
+          b = rd;
+        } else {
+          a = (rd + pi) % (np - 1);
+          b = (rd - pi + np - 1) % (np - 1);
+        }
+        if (a >= n || b >= n) continue;
+        double *x = r + (j.roff + a * j.rld) * ed, *y = r + (j.roff + b * j.rld) * ed;
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+        for (int64_t c = lane; c < j.m; c += 64) {
+          if (ed == 2) {
+            const double xr = x[2 * c], xi = x[2 * c + 1], yr = y[2 * c], yi = y[2 * c + 1];
+            al += xr * xr + xi * xi;
+            be += yr * yr + yi * yi;
+            gr += xr * yr + xi * yi;   // x conj(y)
+            gi += xi * yr - xr * yi;
+          } else {
+            al += x[c] * x[c];
+            be += y[c] * y[c];
+            gr += x[c] * y[c];
+          }
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        const double g = sqrt(gr * gr + gi * gi);
+        if (!(g > 1e-15 * sqrt(al * be)) || al == 0.0 || be == 0.0) continue;
+        const double zeta = (be - al) / (2.0 * g);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        const double pr = gr / g, pim = gi / g;   // e^{i phi}
+        if (lane == 0) rotated = 1;
+        auto rot = [&](double *u, double *v, int64_t len) {
+          for (int64_t c = lane; c < len; c += 64) {
+            if (ed == 2) {
+              const double ur = u[2 * c], ui = u[2 * c + 1], vr = v[2 * c], vi = v[2 * c + 1];
+              const double wr = pr * vr - pim * vi, wi = pr * vi + pim * vr;   // e^{i phi} v
+              u[2 * c] = cs * ur - sn * wr;
+              u[2 * c + 1] = cs * ui - sn * wi;
+              v[2 * c] = sn * ur + cs * wr;
+              v[2 * c + 1] = sn * ui + cs * wi;
+            } else {
+              const double uu = u[c], vv = pr * v[c];
+              u[c] = cs * uu - sn * vv;
+              v[c] = sn * uu + cs * vv;
+            }
+          }
+        };
+        rot(x, y, j.m);
+        rot(e + (j.eoff + a * j.eld) * ed, e + (j.eoff + b * j.eld) * ed, j.eld);
+      }
+      __syncthreads();
+    }
+    const int any = rotated;
+    __syncthreads();
+    if (!any) break;
+  }
+}
+
+// output assembly: dst[o + r * cols + c] = src value at base + r * rs + c * cs
+// (conjugated if conj), one workgroup per output block
+__global__ __launch_bounds__(VT) void assemble_kernel(const SvdOut *__restrict__ jobs, const double *__restrict__ src,
+                                                      double *__restrict__ dst, int ed) {
+  const SvdOut j = jobs[blockIdx.x];
+  const int64_t tot = j.rows * j.cols;
+  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+    const int64_t r = e / j.cols, c = e % j.cols;
+    const int64_t s = j.base + r * j.rs + c * j.cs;
+    dst[(j.dst + e) * ed] = src[s * ed];
+    if (ed == 2) dst[(j.dst + e) * ed + 1] = j.conj ? -src[s * ed + 1] : src[s * ed + 1];
+  }
+}
+
+// s blocks: diagonal k x k matrices from the kept singular values
+__global__ __launch_bounds__(VT) void diag_kernel(const SvdRows *__restrict__ jobs, const double *__restrict__ sv,
+                                                  double *__restrict__ dst) {
+  const SvdRows j = jobs[blockIdx.x];
+  for (int64_t e = threadIdx.x; e < j.k * j.k; e += VT) {
+    const int64_t r = e / j.k, c = e % j.k;
+    dst[j.dst + e] = r == c ? sv[j.src + r] : 0.0;
+  }
+}
+}  // namespace
+
+#define QLG_LAUNCH(kern, n, ...)                                                        \
+  do {                                                                                  \
+    if ((n) > 0) {                                                                      \
+      hipLaunchKernelGGL(kern, dim3((unsigned)(n)), dim3(VT), 0, st, __VA_ARGS__);    \
+      QLG_HIP(hipGetLastError());                                                       \
+    }                                                                                   \
+  } while (0)
+
+void svd_gather(hipStream_t st, const SvdBlk *tab, int64_t n, const double *src, double *dst, int ed) {
+  QLG_LAUNCH(gather_kernel, n, tab, src, dst, ed);
+}
+void svd_normalise(hipStream_t st, const SvdMat *mats, int64_t n, double *x, double *scale, int ed) {
+  QLG_LAUNCH(normalise_kernel, n, mats, x, scale, ed);
+}
+void svd_pad_diag(hipStream_t st, const SvdMat *mats, int64_t n, double *g, int ed) {
+  QLG_LAUNCH(pad_diag_kernel, n, mats, g, ed);
+}
+void svd_reflect(hipStream_t st, const SvdMat *mats, int64_t n, int64_t bmax, double *g, const double *u) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(reflect_kernel, dim3((unsigned)n), dim3(VT), (size_t)bmax * sizeof(double2), st, mats,
+                     reinterpret_cast<double2 *>(g), reinterpret_cast<const double2 *>(u));
+  QLG_HIP(hipGetLastError());
+}
+void svd_reflect_rows(hipStream_t st, const SvdMat *mats, int64_t n, double *e, const double *u) {
+  QLG_LAUNCH(reflect_rows_kernel, n, mats, reinterpret_cast<double2 *>(e), reinterpret_cast<const double2 *>(u));
+}
+void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, const double *x, double *out, int ed) {
+  QLG_LAUNCH(row_norms_kernel, n, mats, x, out, ed);
+}
+void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, const int32_t *perm, const double *f,
+                     const double *src, double *dst, int ed) {
+  QLG_LAUNCH(gather_rows_kernel, n, jobs, perm, f, src, dst, ed);
+}
+void svd_ns_dev(hipStream_t st, const SvdMat *mats, int64_t n, const double *a, const double *mask, double *dev,
+                int ed) {
+  QLG_LAUNCH(ns_dev_kernel, n, mats, a, mask, dev, ed);
+}
+void svd_ns_update(hipStream_t st, const SvdMat *mats, int64_t n, double *p, const double *t, int ed) {
+  QLG_LAUNCH(ns_update_kernel, n, mats, p, t, ed);
+}
+void svd_complete_rows(hipStream_t st, const SvdMat *mats, int64_t n, double *p, double *mask, int ed) {
+  QLG_LAUNCH(complete_rows_kernel, n, mats, p, mask, ed);
+}
+void svd_assemble(hipStream_t st, const SvdOut *jobs, int64_t n, const double *src, double *dst, int ed) {
+  QLG_LAUNCH(assemble_kernel, n, jobs, src, dst, ed);
+}
+void svd_diag(hipStream_t st, const SvdRows *jobs, int64_t n, const double *sv, double *dst) {
+  QLG_LAUNCH(diag_kernel, n, jobs, sv, dst);
+}
+void svd_jacobi_rows(hipStream_t st, const SvdJac *jobs, int64_t n, double *r, double *e, int ed) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(jacobi_rows_kernel, dim3((unsigned)n), dim3(1024), 0, st, jobs, r, e, ed);
+  QLG_HIP(hipGetLastError());
+}
+
+}  // namespace qlg

@@ -12,13 +12,13 @@ Reference behaviour followed:
 
 Everything stays on the GPU: contractions and environment updates go through
 the library's planners, the Krylov solve through ``qlg_lanczos_gs``, and the
-per-sector SVDs through batched FP64 eigen-decompositions of the Gram matrix
-on each sector matrix's smaller side (torch / rocSOLVER).  The truncation
-rule is this package's: pooled singular values, smallest D whose discarded
-weight sum(s^2)/total <= trunc_err, clamped to [Dmin, Dmax].
+block SVD through ``qlg_block_svd``.  The truncation rule is the library's:
+pooled singular values, smallest D whose discarded weight
+sum(s^2)/total <= trunc_err, clamped to [Dmin, Dmax].
 """
 from __future__ import annotations
 
+import ctypes as C
 import sys
 import time
 from dataclasses import dataclass, field
@@ -55,180 +55,6 @@ class SVDResult:
     entropy: float
 
 
-# padded sizes of the batched eigen-decompositions.  The divide-and-conquer
-# solver's tridiagonal reduction is bound by host launch latency: a batch
-# costs ~22 us x its padded order x (1 + 0.045 x batch) on MI355X.  Batches run
-# concurrently on four lanes, but lanes contend for the launch path, so a few
-# coarse buckets beat fine ones (30 ms against 38 ms per D=2000 SVD for 64-wide
-# steps above 512).  Sectors with one row need no solver.
-_BUCKETS = (1, 256, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192)
-_SV_NOISE = 1e-6   # singular values at or below this fraction of their sector's largest count as zero
-
-
-def _bucket(n: int) -> int:
-    for b in _BUCKETS:
-        if n <= b:
-            return b
-    return -(-n // 4096) * 4096
-
-
-class _SvdStructure:
-    """Sector grouping of a tensor's block structure for a block SVD, and the
-    flat gather index that places every element of the tensor into padded,
-    per-size-bucket batches of sector matrices.  Each sector matrix is stored
-    with its shorter side first (transposed when it has more rows than
-    columns), so its Gram matrix on that side is one batched product.  The
-    host builds a per-block table; the per-element index is computed on the
-    device from it."""
-
-    def __init__(self, t: BlockTensor, ldims: int, ldiv):
-        left, right = t.axes[:ldims], t.axes[ldims:]
-        nb, qlen = len(t.coords), len(ldiv)
-        cs = np.asarray(t.coords, dtype=np.int64).reshape(nb, t.rank)
-        # per block: left charge (the sector), row / column extents, and the
-        # left / right coordinates as mixed-radix keys (their integer order is
-        # the tuples' lexicographic order)
-        q = np.tile(np.asarray(ldiv, dtype=np.int64), (nb, 1))
-        nr, nc = np.ones(nb, np.int64), np.ones(nb, np.int64)
-        lkey, rkey = np.zeros(nb, np.int64), np.zeros(nb, np.int64)
-        lrad = rrad = 1
-        for k, ax in enumerate(t.axes):
-            col = cs[:, k]
-            deg = np.asarray(ax.degs, dtype=np.int64)[col]
-            if k < ldims:
-                sg = -1 if ax.dir == OUT else 1
-                q += sg * np.asarray(ax.qns, dtype=np.int64).reshape(ax.nsect, qlen)[col]
-                nr *= deg
-                lkey, lrad = lkey * ax.nsect + col, lrad * ax.nsect
-            else:
-                nc *= deg
-                rkey, rrad = rkey * ax.nsect + col, rrad * ax.nsect
-        qs, gid = np.unique(q, axis=0, return_inverse=True)
-        gid = gid.reshape(-1)
-        ng = len(qs)
-
-        def offsets(key, rad, ext):
-            # distinct (sector, key) pairs in sorted order; each pair's offset
-            # within its sector (exclusive prefix sum of extents) and each
-            # sector's total extent
-            pk, first, inv = np.unique(gid * rad + key, return_index=True, return_inverse=True)
-            pg, pext = pk // rad, ext[first]
-            ex = np.cumsum(pext) - pext
-            start = np.searchsorted(pg, np.arange(ng))
-            return first, pg, pext, ex - ex[start][pg], np.add.reduceat(pext, start), inv.reshape(-1)
-
-        rfirst, rg, rext, roff, Rs, rinv = offsets(lkey, lrad, nr)
-        cfirst, cg, cext, coff, Cs, cinv = offsets(rkey, rrad, nc)
-        self.qs = [tuple(int(v) for v in x) for x in qs]
-        self.sect = [dict(q=qq, rows={}, cols={}, rks=[], cks=[], ro={}, co={}, R=int(Rs[g]), C=int(Cs[g]),
-                          tr=bool(Rs[g] > Cs[g]), n=int(min(Rs[g], Cs[g])), m=int(max(Rs[g], Cs[g])))
-                     for g, qq in enumerate(self.qs)]
-        for i, g, e, o in zip(rfirst.tolist(), rg.tolist(), rext.tolist(), roff.tolist()):
-            sc, lk = self.sect[g], t.coords[i][:ldims]
-            sc["rks"].append(lk)
-            sc["rows"][lk], sc["ro"][lk] = e, o
-        for i, g, e, o in zip(cfirst.tolist(), cg.tolist(), cext.tolist(), coff.tolist()):
-            sc, rk = self.sect[g], t.coords[i][ldims:]
-            sc["cks"].append(rk)
-            sc["cols"][rk], sc["co"][rk] = e, o
-        # buckets of sectors by padded short side; slot j of bucket b.  A
-        # bucket is padded only to its largest member (a multiple of 16), not
-        # to the bucket's bound: the eigensolver's time grows with the order.
-        raw: Dict[int, List[int]] = {}
-        for si, sc in enumerate(self.sect):
-            raw.setdefault(_bucket(sc["n"]), []).append(si)
-        self.buckets: Dict[int, List[int]] = {}
-        for b, sis in raw.items():
-            if b > 1:
-                b = -(-max(self.sect[si]["n"] for si in sis) // 16) * 16
-            self.buckets[b] = sis
-        self.mpad = {b: max(16, -(-max(self.sect[si]["m"] for si in sis) // 16) * 16)
-                     for b, sis in self.buckets.items()}
-        # per-block table: data offset, rows, cols, batch base, row/col origin in
-        # the sector matrix, transposed flag, padded long side
-        sbase = np.zeros(ng, np.int64)
-        smp = np.zeros(ng, np.int64)
-        base = 0
-        self.bucket_base = {}
-        for b, sis in self.buckets.items():
-            mp = self.mpad[b]
-            self.bucket_base[b] = base
-            for j, si in enumerate(sis):
-                sbase[si] = base + j * b * mp
-                smp[si] = mp
-            base += len(sis) * b * mp
-        tr = np.array([sc["tr"] for sc in self.sect], dtype=np.int64)
-        tab = np.stack([np.asarray(t.offsets, dtype=np.int64).reshape(nb), nr, nc, sbase[gid], roff[rinv],
-                        coff[cinv], tr[gid], smp[gid]], axis=1)
-        self.total = base
-        dev = t.data.device
-        sizes = tab[:, 1] * tab[:, 2]
-        ntot = int(sizes.sum())
-        T = torch.from_numpy(np.ascontiguousarray(tab.T)).to(dev)           # [8, nblocks]
-        size = T[1] * T[2]
-        blk = torch.repeat_interleave(torch.arange(nb, device=dev), size, output_size=ntot)
-        Tb = T[:, blk]                                                        # [8, elements]
-        loc = torch.arange(ntot, device=dev) - (torch.cumsum(size, 0) - size)[blk]
-        r = torch.div(loc, Tb[2], rounding_mode="floor")
-        rr, cc = Tb[4] + r, Tb[5] + (loc - r * Tb[2])
-        pos = Tb[3] + torch.where(Tb[6] > 0, cc * Tb[7] + rr, rr * Tb[7] + cc)
-        dst = torch.full((t.data.numel(),), base, dtype=torch.int64, device=dev)   # base: one trash slot
-        dst[Tb[0] + loc] = pos
-        self.dst = dst
-
-
-_SVD_CACHE: Dict[tuple, _SvdStructure] = {}
-
-
-def _svd_structure(t: BlockTensor, ldims: int, ldiv) -> _SvdStructure:
-    key = (tuple(t.axes), tuple(t.coords), tuple(t.offsets), ldims, tuple(ldiv), t.data.numel())
-    st = _SVD_CACHE.get(key)
-    if st is None:
-        if len(_SVD_CACHE) > 256:
-            _SVD_CACHE.clear()
-        st = _SVD_CACHE[key] = _SvdStructure(t, ldims, ldiv)
-    return st
-
-
-_REFLECTORS: Dict[tuple, torch.Tensor] = {}
-
-
-def _reflectors(b: int, dtype, device, attempt: int = 0) -> torch.Tensor:
-    """Two fixed random unit vectors u_1, u_2 of length b ([2, b, 1]); the
-    Householder reflectors H_i = I - 2 u_i u_i^H are Hermitian and unitary, and
-    Q = H_2 H_1 turns a block-diagonal Hermitian matrix into a dense one."""
-    key = (b, dtype, str(device), attempt)
-    u = _REFLECTORS.get(key)
-    if u is None:
-        if len(_REFLECTORS) > 512:
-            _REFLECTORS.clear()
-        gen = torch.Generator(device="cpu").manual_seed(4321 + 7919 * attempt + b)
-        z = torch.complex(torch.randn(2, b, 1, generator=gen, dtype=torch.float64),
-                          torch.randn(2, b, 1, generator=gen, dtype=torch.float64))
-        z = z / torch.linalg.vector_norm(z, dim=1, keepdim=True)
-        u = _REFLECTORS[key] = z.to(device=device, dtype=dtype)
-    return u
-
-
-def _rotate_hermitian(G: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
-    """Q G Q^H for Q = H_2 H_1 and a batch G [B, n, n] of Hermitian matrices:
-    H G H = G - 2 u w^H - 2 w u^H + 4 (u^H w) u u^H with w = G u."""
-    for i in range(u.shape[0]):
-        ui = u[i]
-        w = G @ ui                                   # [B, n, 1]
-        sc = (ui.mH @ w).real                        # [B, 1, 1]
-        G = G - 2 * ui @ w.mH - 2 * w @ ui.mH + 4 * sc * (ui @ ui.mH)
-    return G
-
-
-def _unrotate_vectors(V: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
-    """Q^H V = H_1 H_2 V: eigenvectors of Q G Q^H back to those of G."""
-    for i in reversed(range(u.shape[0])):
-        ui = u[i]
-        V = V - 2 * ui @ (ui.mH @ V)
-    return V
-
-
 def eigh_batched(mats: List[torch.Tensor]) -> List[Tuple[torch.Tensor, torch.Tensor]]:
     """Eigen-decompositions of batches of Hermitian matrices ([B, n, n] each, all
     of one dtype, n > 1) through ``qlg_eigh_batched``: one library call for all
@@ -251,292 +77,67 @@ def eigh_batched(mats: List[torch.Tensor]) -> List[Tuple[torch.Tensor, torch.Ten
     return [(w, A.mT.conj()) for w, A in zip(ws, mats)]
 
 
-def _eigh_grams(grams: Dict[int, torch.Tensor]) -> Dict[int, Tuple[torch.Tensor, torch.Tensor]]:
-    """Eigen-decompositions of the padded Gram batches of a block SVD.
-
-    Complex batches are rotated by a fixed unitary Q = H_2 H_1 (two random
-    Householder reflectors), Q G Q^H, before the solve and the vectors rotated
-    back: the complex divide-and-conquer solver returns NaN vectors when the
-    tridiagonal reduction meets exactly-zero sub-columns, which block-diagonal
-    (padded or reducible) Gram matrices produce.  A batch that still comes back
-    with NaN is solved again with other reflectors; a third failure raises."""
-    out, todo = {}, []
-    for b, G in grams.items():
-        if b == 1:
-            out[b] = (G[:, :, 0].real.contiguous(), torch.ones_like(G))
-        else:
-            todo.append(b)
-    if not todo:
-        return out
-    if not grams[todo[0]].is_complex():
-        for b, r in zip(todo, eigh_batched([grams[b].clone() for b in todo])):
-            out[b] = r
-        return out
-    for attempt in range(3):
-        us = {b: _reflectors(b, grams[b].dtype, grams[b].device, attempt) for b in todo}
-        res = eigh_batched([_rotate_hermitian(grams[b], us[b]).contiguous() for b in todo])
-        left = []
-        for b, (w, V) in zip(todo, res):
-            if bool(torch.isnan(V).any() or torch.isnan(w).any()):
-                left.append(b)
-            else:
-                out[b] = (w, _unrotate_vectors(V, us[b]))
-        todo = left
-        if not todo:
-            return out
-    raise RuntimeError(f"block SVD: the eigensolver returned NaN for Gram batches of order {todo}")
-
-
-def _complete_rows(A: torch.Tensor, bad: torch.Tensor) -> torch.Tensor:
-    """Replace the rows of A flagged in ``bad`` by unit vectors orthogonal to the others."""
-    k, n = A.shape
-    good = A[~bad]
-    gen = torch.Generator(device="cpu").manual_seed(1234)
-    R = torch.randn(int(bad.sum()), n, generator=gen, dtype=torch.float64).to(A.device, A.dtype)
-    Q, _ = torch.linalg.qr(torch.cat([good, R]).mT)
-    out = A.clone()
-    out[bad] = Q.mT[good.shape[0]:]
-    return out
-
-
-def _gather_runs(src: torch.Tensor, starts, sizes, dev) -> torch.Tensor:
-    """Concatenation of the runs src[starts[i] : starts[i] + sizes[i]] (one device gather)."""
-    n = torch.tensor(sizes, dtype=torch.int64)
-    total = int(n.sum())
-    if total == 0:
-        return torch.zeros(1, dtype=src.dtype, device=dev)
-    shift = (torch.tensor(starts, dtype=torch.int64) - (torch.cumsum(n, 0) - n)).to(dev)
-    blk = torch.repeat_interleave(torch.arange(len(sizes), device=dev), n.to(dev), output_size=total)
-    return src[shift[blk] + torch.arange(total, device=dev)]
-
-
-def _gather_cols(src: torch.Tensor, starts, rows, cols, ld, dev) -> torch.Tensor:
-    """Concatenation of the row-major [rows[i] x cols[i]] sub-matrices starting at
-    src[starts[i]] with leading dimension ld[i] (one device gather)."""
-    size = [r * c for r, c in zip(rows, cols)]
-    n = torch.tensor(size, dtype=torch.int64)
-    total = int(n.sum())
-    if total == 0:
-        return torch.zeros(1, dtype=src.dtype, device=dev)
-    T = torch.tensor([starts, cols, ld, list(np.cumsum(size) - np.array(size))], dtype=torch.int64).to(dev)
-    blk = torch.repeat_interleave(torch.arange(len(size), device=dev), n.to(dev), output_size=total)
-    loc = torch.arange(total, device=dev) - T[3][blk]
-    c = T[1][blk]
-    r = torch.div(loc, c, rounding_mode="floor")
-    return src[T[0][blk] + r * T[2][blk] + (loc - r * c)]
-
-
-def _gather_strided(src: torch.Tensor, tab: np.ndarray, dev) -> torch.Tensor:
-    """Concatenation of the [rows x cols] matrices (row-major) whose element
-    (r, c) is src[base + r * rs + c * cs], one per row (rows, cols, base, rs,
-    cs) of ``tab`` (one device gather)."""
-    size = tab[:, 0] * tab[:, 1]
-    total = int(size.sum())
-    if total == 0:
-        return torch.zeros(1, dtype=src.dtype, device=dev)
-    T = torch.from_numpy(np.ascontiguousarray(np.concatenate([tab, (np.cumsum(size) - size)[:, None]], axis=1).T)).to(dev)
-    blk = torch.repeat_interleave(torch.arange(len(tab), device=dev), T[0] * T[1], output_size=total)
-    loc = torch.arange(total, device=dev) - T[5][blk]
-    c = T[1][blk]
-    r = torch.div(loc, c, rounding_mode="floor")
-    return src[T[2][blk] + r * T[3][blk] + (loc - r * c) * T[4][blk]]
-
-
 def block_svd(t: BlockTensor, ldims: int, ldiv, trunc_err: float, Dmin: int, Dmax: int) -> SVDResult:
-    """Block SVD with truncation, through FP64 eigen-decompositions of each
-    sector matrix's Gram matrix on its shorter side, batched by size bucket.
+    """The reference's SVD(t, ldims, div, trunc_err, Dmin, Dmax, u, s, vt,
+    actual_trunc_err, D) (two_site_update_finite_vmps_impl.h:302-307) through
+    ``qlg_block_svd``: sector matrices gathered on the GPU, FP64 Gram
+    eigen-decompositions, back-projection, refinement of kept values below
+    1e-6 of their sector's largest, Newton-Schulz orthonormalisation and the
+    assembly of u, s and vt, all in the library.  Truncation (the library's
+    rule): pooled singular values, smallest D whose discarded weight
+    sum(s^2)/total <= trunc_err, clamped to [Dmin, Dmax].  s is returned in
+    t's dtype (its values are real); ``entropy`` is the von Neumann entropy of
+    the kept weights."""
+    dev = t.data.device
+    ctx = default_context(dev.index or 0)
+    lib = ctx.lib
+    qlen = t.qn_len
+    desc = t.descriptor()
+    ld = (C.c_int32 * qlen)(*[int(v) for v in ldiv])
+    h = C.c_void_p()
+    torch.cuda.synchronize()
+    ctx.check(lib.qlg_block_svd(ctx.h, C.byref(desc), int(ldims), ld, float(trunc_err), int(Dmin), int(Dmax),
+                                C.byref(h)))
+    try:
+        err, D, ns = C.c_double(), C.c_int64(), C.c_int64()
+        ctx.check(lib.qlg_svd_result(h, C.byref(err), C.byref(D), C.byref(ns)))
+        qn = (C.c_int32 * max(ns.value * qlen, 1))()
+        deg = (C.c_int64 * max(ns.value, 1))()
+        ctx.check(lib.qlg_svd_bond(h, qn, deg))
+        mid_out = Axis.make([(tuple(qn[i * qlen:(i + 1) * qlen]), deg[i]) for i in range(ns.value)], OUT)
+        mid_in = mid_out.inverse()
 
-    Truncation (this package's rule): pool the singular values of all sectors,
-    keep the smallest D whose discarded weight sum(s^2)/total <= trunc_err,
-    clamped to [Dmin, Dmax].  The Gram route resolves singular values to an
-    absolute ~sqrt(n eps) of the sector's largest, so values at or below 1e-6
-    of it (Schmidt weight <= 1e-12 relative) are returned as zero, and the
-    singular vectors of kept zero values are completed to an orthonormal set."""
-    left, right = t.axes[:ldims], t.axes[ldims:]
-    dev, dt = t.data.device, t.data.dtype
-    st = _svd_structure(t, ldims, ldiv)
-    flat = torch.zeros(st.total + 1, dtype=dt, device=dev)
-    flat[st.dst] = t.data
-    per = {}   # bucket -> (E [B, b, b] descending eigenvectors, S [B, b], P = E^T X / S [B, b, mpad])
-    svals = []
-    Xs, grams, scales = {}, {}, {}
-    for b, sis in st.buckets.items():
-        mp, B = st.mpad[b], len(sis)
-        X = flat[st.bucket_base[b]:st.bucket_base[b] + B * b * mp].view(B, b, mp)
-        # each sector matrix is scaled to unit Frobenius norm: the solver's
-        # tolerances are relative to the (padded) matrix norm, and the padding
-        # below must sit at the sector's own scale, not at that of the whole state
-        nrm = torch.linalg.vector_norm(X, dim=(1, 2))
-        nrm = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
-        X = X / nrm[:, None, None].to(dt)
-        scales[b] = nrm
-        G = X @ X.mT.conj()
-        ns = torch.tensor([st.sect[si]["n"] for si in sis], device=dev)
-        pad = torch.arange(b, device=dev)[None, :] >= ns[:, None]
-        # padded rows separate below the spectrum (distinct values for complex
-        # data, whose solver is given a rotated, dense matrix)
-        padv = -1.0 - torch.arange(b, device=dev, dtype=torch.float64) / b if G.is_complex() else -1.0
-        G.diagonal(dim1=1, dim2=2)[pad] = (padv.expand(len(sis), b)[pad].to(G.dtype) if G.is_complex() else padv)
-        Xs[b], grams[b] = X, G
-    eig = _eigh_grams(grams)
-    for b in st.buckets:
-        X = Xs[b]
-        w, V = eig[b]
-        E = V.flip(-1)                                   # eigenvectors, descending eigenvalue
-        R = E.mT.conj() @ X                              # row j = s_j v_j^H (un-normalised)
-        # Singular values from the back-projected rows, s_j = |X^H e_j|: relative
-        # error ~ eps s_max / s_j, against eps (s_max / s_j)^2 for sqrt(lambda_j).
-        # Re-sorted descending within each sector (near-degenerate small values
-        # may swap).  Values at or below _SV_NOISE of the sector's largest count
-        # as zero; their singular vectors are completed to an orthonormal set below.
-        S = torch.linalg.vector_norm(R, dim=-1)
-        # padded positions (the last b - n eigenvectors) stay last
-        nsec = torch.tensor([st.sect[si]["n"] for si in st.buckets[b]], device=dev)
-        S = torch.where(torch.arange(S.shape[1], device=dev)[None, :] < nsec[:, None], S, torch.full_like(S, -1.0))
-        S, order = torch.sort(S, dim=-1, descending=True)
-        S = S.clamp_min(0.0)
-        E = torch.gather(E, 2, order[:, None, :].expand(E.shape))
-        R = torch.gather(R, 1, order[:, :, None].expand(R.shape))
-        S = torch.where(S > _SV_NOISE * S[:, :1], S, torch.zeros_like(S))
-        Sinv = torch.where(S > 0, 1.0 / torch.where(S > 0, S, torch.ones_like(S)), torch.zeros_like(S))
-        P = R * Sinv[..., None].to(dt)                   # unit rows; zero rows for zero values
-        S = S * scales[b][:, None]                       # back to the tensor's scale
-        per[b] = (E, S, P)
-        svals.append(S)
-    # Orthogonality between rows degrades as eps s_max^2 / (s_i s_j), ~1e-4 at
-    # the noise threshold: Newton-Schulz steps toward the polar factor,
-    # P <- 1.5 P - 0.5 (P P^H) P, restore it quadratically.  The reconstruction
-    # moves by at most ~eps s_max^2 / s.  Zero rows stay zero.  The buckets run
-    # in lock step, one host check per round for all of them; each stops at
-    # its own deviation < 1e-13.
-    masks = {b: (per[b][1] > 0).to(dt) for b in st.buckets}
-    active = list(st.buckets)
-    for _ in range(8):
-        if not active:
-            break
-        PPs = {b: per[b][2] @ per[b][2].mH for b in active}
-        devs = torch.stack([(PPs[b] - torch.diag_embed(masks[b])).abs().amax() for b in active]).tolist()
-        still = []
-        for b, odev in zip(active, devs):
-            if odev < 1e-13:
-                continue
-            if odev > 0.5:
-                raise RuntimeError(f"block SVD: singular vectors too far from orthonormal ({odev:.2e})")
-            E, S, P = per[b]
-            per[b] = (E, S, 1.5 * P - 0.5 * (PPs[b] @ P))
-            still.append(b)
-        active = still
-    # one host transfer for the truncation decision
-    flat_sv = torch.cat([s_.reshape(-1) for s_ in svals]).double().cpu().numpy()
-    sv_all, pos = [], 0
-    for s_ in svals:
-        sv_all.append(flat_sv[pos:pos + s_.numel()].reshape(s_.shape))
-        pos += s_.numel()
-    sv, owner, sv_host = [], [], {}
-    for (b, sis), arr in zip(st.buckets.items(), sv_all):
-        for j, si in enumerate(sis):
-            n = st.sect[si]["n"]
-            sv_host[si] = arr[j, :n]
-            sv.append(arr[j, :n])
-            owner.append(np.full(n, si))
-    sv = np.concatenate(sv)
-    owner = np.concatenate(owner)
-    order = np.argsort(-sv, kind="stable")
-    sq = sv[order] ** 2
-    total = float(sq.sum())
-    tail = np.concatenate([np.cumsum(sq[::-1])[::-1], [0.0]])
-    D = len(sv)
-    for k in range(len(sv) + 1):
-        if tail[k] / total <= trunc_err:
-            D = k
-            break
-    D = max(min(max(D, min(Dmin, len(sv))), Dmax, len(sv)), 1)
-    err = float(tail[D] / total)
-    keep = np.bincount(owner[order[:D]], minlength=len(st.sect))
-    smax = float(sv[order[0]]) if len(sv) else 0.0
-    kept_si = [si for si in range(len(st.sect)) if keep[si] > 0]
-    mid_out = Axis.make([(st.sect[si]["q"], int(keep[si])) for si in kept_si], OUT)
-    mid_in = mid_out.inverse()
-    slot = {}
-    for b, sis in st.buckets.items():
-        for j, si in enumerate(sis):
-            slot[si] = (b, j)
-    # Per kept sector, U (R x k) and V^H (k x C), row-major, are gathered from
-    # the buckets' E and P into two flat buffers (one device gather each), and
-    # its k singular values into a third; the blocks of u and vt are then cut
-    # out of them by one device gather each (u blocks are row ranges of U, vt
-    # blocks column ranges of V^H), and s is scattered onto its diagonal blocks.
-    #   X = M (R <= C):   U = E[:n, :k],        V^H = P[:k, :m]
-    #   X = M^T (R > C):  E = conj(V), so V^H = E[:n, :k]^T and U = M conj(E) / S = P[:k, :m]^T
-    parts, sparts, ebase, pbase, sbase_ = [], [], {}, {}, {}
-    off = soff = 0
-    for b in st.buckets:
-        E, S, P = per[b]
-        ebase[b], pbase[b], sbase_[b] = off, off + E.numel(), soff
-        parts += [E.reshape(-1), P.reshape(-1)]
-        sparts.append(S.reshape(-1))
-        off += E.numel() + P.numel()
-        soff += S.numel()
-    utab = np.zeros((len(kept_si), 5), dtype=np.int64)
-    vtab = np.zeros((len(kept_si), 5), dtype=np.int64)
-    stab = np.zeros((len(kept_si), 5), dtype=np.int64)
-    ubase, vbase, ks = {}, {}, {}
-    uoff = voff = 0
-    for mi, si in enumerate(kept_si):
-        sc, k = st.sect[si], int(keep[si])
-        b, j = slot[si]
-        n, m, mp = sc["n"], sc["m"], st.mpad[b]
-        eb, pb = ebase[b] + j * b * b, pbase[b] + j * b * mp
-        if sc["tr"]:
-            utab[mi], vtab[mi] = (m, k, pb, 1, mp), (k, n, eb, 1, b)
-        else:
-            utab[mi], vtab[mi] = (n, k, eb, b, 1), (k, m, pb, mp, 1)
-        stab[mi] = (1, k, sbase_[b] + j * b, 0, 1)
-        ubase[si], vbase[si], ks[si] = uoff, voff, k
-        uoff += sc["R"] * k
-        voff += k * sc["C"]
-    src = torch.cat(parts)
-    Uflat = _gather_strided(src, utab, dev).to(dt)
-    Vflat = _gather_strided(src, vtab, dev).to(dt)
-    Sflat = _gather_strided(torch.cat(sparts), stab, dev)
-    for si in kept_si:
-        k = ks[si]
-        if sv_host[si][k - 1] != 0.0:
-            continue
-        # kept zero singular values: complete their singular vectors
-        sc = st.sect[si]
-        b, j = slot[si]
-        bad = per[b][1][j, :k] == 0
-        if sc["tr"]:
-            U = Uflat[ubase[si]:ubase[si] + sc["R"] * k].view(sc["R"], k)
-            U.copy_(_complete_rows(U.mT.contiguous(), bad).mT)
-        else:
-            Vh = Vflat[vbase[si]:vbase[si] + k * sc["C"]].view(k, sc["C"])
-            Vh.copy_(_complete_rows(Vh.contiguous(), bad))
-    # u blocks (lk, mi): rows [ro, ro + rows) of U -> one contiguous run each
-    ublk = sorted((lk + (mi,), si, lk) for mi, si in enumerate(kept_si) for lk in st.sect[si]["rks"])
-    u_start = [ubase[si] + st.sect[si]["ro"][lk] * ks[si] for _, si, lk in ublk]
-    u_size = [st.sect[si]["rows"][lk] * ks[si] for _, si, lk in ublk]
-    u = BlockTensor(list(left) + [mid_out], [c for c, _, _ in ublk],
-                    _gather_runs(Uflat, u_start, u_size, dev))
-    # vt blocks (mi, rk): columns [co, co + cols) of V^H (k x C)
-    vblk = sorted(((mi,) + rk, si, rk) for mi, si in enumerate(kept_si) for rk in st.sect[si]["cks"])
-    vt = BlockTensor([mid_in] + list(right), [c for c, _, _ in vblk],
-                     _gather_cols(Vflat, [vbase[si] + st.sect[si]["co"][rk] for _, si, rk in vblk],
-                                  [ks[si] for _, si, _ in vblk], [st.sect[si]["cols"][rk] for _, si, rk in vblk],
-                                  [st.sect[si]["C"] for _, si, _ in vblk], dev))
-    # s: diagonal k x k blocks in sector order
-    kk = [ks[si] for si in kept_si]
-    sdata = torch.zeros(max(sum(k * k for k in kk), 1), dtype=dt, device=dev)
-    sb = np.concatenate([[0], np.cumsum([k * k for k in kk])[:-1]]).astype(np.int64)
-    diag_pos = torch.from_numpy(np.concatenate([sb[i] + np.arange(k) * (k + 1) for i, k in enumerate(kk)])).to(dev)
-    sdata[diag_pos] = Sflat.to(dt)
-    s = BlockTensor([mid_in, mid_out], [(mi, mi) for mi in range(len(kept_si))], sdata)
-    pr = np.sort(sv)[::-1][:D] ** 2
-    pr = pr / pr.sum()
+        def fetch(which, axes, dtype):
+            nb, size = C.c_int64(), C.c_int64()
+            ctx.check(lib.qlg_svd_tensor(h, which, C.byref(nb), C.byref(size), None, None, None))
+            r = len(axes)
+            coords = (C.c_int32 * max(nb.value * r, 1))()
+            offs = (C.c_int64 * max(nb.value, 1))()
+            data = C.c_void_p()
+            ctx.check(lib.qlg_svd_tensor(h, which, None, None, coords, offs, C.byref(data)))
+            buf = torch.empty(max(size.value, 1), dtype=dtype, device=dev)
+            nbytes = size.value * buf.element_size()
+            if nbytes:
+                ctx.check(lib.qlg_memcpy_d2d(ctx.h, C.c_void_p(buf.data_ptr()), data, nbytes))
+            cl = [tuple(coords[b * r:(b + 1) * r]) for b in range(nb.value)]
+            return BlockTensor(axes, cl, buf, list(offs[:nb.value]))
+
+        dt = t.data.dtype
+        u = fetch(0, list(t.axes[:ldims]) + [mid_out], dt)
+        s = fetch(1, [mid_in, mid_out], torch.float64)
+        vt = fetch(2, [mid_in] + list(t.axes[ldims:]), dt)
+    finally:
+        lib.qlg_svd_destroy(h)
+    # kept singular values: the diagonals of s's blocks
+    diag = torch.cat([s.data[o:o + k * k].view(k, k).diagonal()
+                      for (mi, _), o, k in zip(s.coords, s.offsets, mid_out.degs)]) if s.coords else s.data[:0]
+    pr = diag.double() ** 2
+    pr = pr / pr.sum() if float(pr.sum()) > 0 else pr
     pr = pr[pr > 0]
-    ee = float(-(pr * np.log(pr)).sum())
-    return SVDResult(u, s, vt, err, D, ee)
+    ee = float(-(pr * torch.log(pr)).sum()) if pr.numel() else 0.0
+    if dt.is_complex:
+        s = BlockTensor(s.axes, s.coords, s.data.to(dt), s.offsets)
+    return SVDResult(u, s, vt, float(err.value), int(D.value), ee)
 
 
 # ---------------------------------------------------------------------------
