@@ -130,7 +130,7 @@ def captured_matvec(drv, site, modes, reps=20):
 
 
 def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 1000), D=2000, max_converge=12,
-                             conv_tol=1e-8, nsweeps=2):
+                             conv_tol=1e-8, nsweeps=2, with_matvec=True):
     """Config 3: the 4x16 Hubbard cylinder (t=1, U=8, 56 electrons, Sz=0, the
     reference benchmark's initial product state, benchmark/ultradmrg_hubbard2d.cpp:130-248;
     Lanczos 1e-9 / 100, truncation 1e-8) ramped through ``ramp`` (one sweep
@@ -157,13 +157,19 @@ def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 
     drv = TwoSiteVMPS(direct_state_mps(m, labs), M.build_mpo(m),
                       SweepParams(1, ramp[0], ramp[0], 1e-8, lp, verbose=False))
     t0 = time.perf_counter()
+
+    def progress(what, e):
+        print(f"[bench] {what}: E = {e:.10f} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
     for Dr in ramp:
         drv.p = SweepParams(1, Dr, Dr, 1e-8, lp, verbose=False)
-        drv.sweep()
+        progress(f"ramp sweep D={Dr}", drv.sweep())
     drv.p = SweepParams(1, D, D, 1e-8, lp, verbose=False)
     conv = [drv.sweep()]
+    progress(f"D={D} sweep 1", conv[-1])
     while len(conv) < max_converge:
         conv.append(drv.sweep())
+        progress(f"D={D} sweep {len(conv)}", conv[-1])
         if abs(conv[-1] - conv[-2]) <= conv_tol * abs(conv[-1]):
             break
     ramp_s = time.perf_counter() - t0
@@ -182,6 +188,7 @@ def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 
         for _ in range(nsweeps):
             es.append(drv.sweep())
             secs.append(drv.stats.sweep_seconds[-1])
+            progress(f"timed {prec} sweep", es[-1])
         torch.cuda.synchronize()
         out[prec] = {"sweep_s": secs, "energy": es[-1], "sweep_energies": es, "lanczos_s": st.lanczos_s, "svd_s": st.svd_s,
                      "env_s": st.env_s, "contract_s": st.contract_s, "plan_s": st.plan_s, "matvecs": st.matvecs,
@@ -193,9 +200,10 @@ def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 
     out["alt"] = {m: {"speedup": sum(out["fp64"]["sweep_s"]) / sum(out[m]["sweep_s"]),
                       "rel_dE": abs(out[m]["energy"] - e64) / abs(e64)} for m in alt}
     # the matvec on the converged state's centre tensors
-    drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
-    drv.p = SweepParams(1, D, D, 1e-8, lp, verbose=False)
-    out["matvec_real"] = captured_matvec(drv, n // 2 - 1, ("fp64", split) + tuple(alt))
+    if with_matvec:
+        drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
+        drv.p = SweepParams(1, D, D, 1e-8, lp, verbose=False)
+        out["matvec_real"] = captured_matvec(drv, n // 2 - 1, ("fp64", split) + tuple(alt))
     return out
 
 
@@ -411,6 +419,8 @@ def main():
     ap.add_argument("--no-tdvp", action="store_true")
     ap.add_argument("--no-chain", action="store_true", help="skip the L=100 Heisenberg ramp")
     ap.add_argument("--tdvp-steps", type=int, default=3, help="TDVP steps timed per form (config 5 runs 40)")
+    ap.add_argument("--sweep-profile", action="store_true",
+                    help="only the Hubbard ramp and one D=2000 sweep per precision (for a profiler run)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this library does not shard the matvec across GPUs; "
@@ -418,6 +428,11 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.sweep_profile:
+        # one sweep of each kind at D=2000 after the ramp, for a profiler run
+        out = hubbard_sweep_comparison(split=a.split, alt=(), max_converge=1, nsweeps=1, with_matvec=False)
+        print(json.dumps({"sweep_profile": out}), flush=True)
         return
 
     import torch
