@@ -130,15 +130,17 @@ def captured_matvec(drv, site, modes, reps=20):
 
 
 def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 1000), D=2000, max_converge=12,
-                             conv_tol=1e-8, nsweeps=2, with_matvec=True):
+                             conv_tol=1e-8, nsweeps=2, with_matvec=True, dmrg_form=True):
     """Config 3: the 4x16 Hubbard cylinder (t=1, U=8, 56 electrons, Sz=0, the
     reference benchmark's initial product state, benchmark/ultradmrg_hubbard2d.cpp:130-248;
     Lanczos 1e-9 / 100, truncation 1e-8) ramped through ``ramp`` (one sweep
     each), then swept at D=2000 until two consecutive sweep energies agree to
     ``conv_tol`` relative (at most ``max_converge`` sweeps, the reference's 12 per
     D).  From that converged state ``nsweeps`` sweeps are timed in FP64, in the
-    split mode and in the ``alt`` modes; the matvec at the chain centre is then
-    timed on the captured converged tensors (``matvec_real``)."""
+    split mode and in the ``alt`` modes; then one sweep per precision through
+    the DMRG-form executor (FiniteDMRG's path, MatReprMPO and block-operator
+    groups, ``dmrg_form``); the matvec at the chain centre is then timed on the
+    captured converged tensors (``matvec_real``)."""
     import torch
     from ultradmrg_amd import mpo as M
     from ultradmrg_amd.solver import LanczosParams
@@ -199,6 +201,24 @@ def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 500, 
     out["rel_dE"] = abs(es - e64) / abs(e64)
     out["alt"] = {m: {"speedup": sum(out["fp64"]["sweep_s"]) / sum(out[m]["sweep_s"]),
                       "rel_dE": abs(out[m]["energy"] - e64) / abs(e64)} for m in alt}
+    if dmrg_form:
+        from ultradmrg_amd.dmrg import DMRGExecutor, mat_repr_mpo
+        mro = mat_repr_mpo(drv.mpo)
+        out["dmrg_form"] = {}
+        for prec in ("fp64", split):
+            t = time.perf_counter()
+            ex = DMRGExecutor(list(saved[0]), mro, SweepParams(1, D, D, 1e-8, lp, precision=prec, verbose=False))
+            init_s = time.perf_counter() - t
+            e = ex.sweep()
+            progress(f"DMRG-form {prec} sweep", e)
+            st = ex.stats
+            out["dmrg_form"][prec] = {"sweep_s": st.sweep_seconds, "block_op_init_s": init_s, "energy": e,
+                                      "lanczos_s": st.lanczos_s, "svd_s": st.svd_s, "block_op_update_s": st.env_s,
+                                      "contract_s": st.contract_s, "plan_s": st.plan_s, "matvecs": st.matvecs}
+        d64 = out["dmrg_form"]["fp64"]
+        d64["rel_dE_vs_vmps_form"] = abs(d64["energy"] - out["fp64"]["sweep_energies"][0]) / abs(d64["energy"])
+        out["dmrg_form"]["speedup"] = d64["sweep_s"][0] / out["dmrg_form"][split]["sweep_s"][0]
+        out["dmrg_form"]["rel_dE"] = abs(out["dmrg_form"][split]["energy"] - d64["energy"]) / abs(d64["energy"])
     # the matvec on the converged state's centre tensors
     if with_matvec:
         drv.mps, drv.lenvs, drv.renvs = list(saved[0]), list(saved[1]), list(saved[2])
@@ -431,7 +451,8 @@ def main():
         return
     if a.sweep_profile:
         # one sweep of each kind at D=2000 after the ramp, for a profiler run
-        out = hubbard_sweep_comparison(split=a.split, alt=(), max_converge=1, nsweeps=1, with_matvec=False)
+        out = hubbard_sweep_comparison(split=a.split, alt=(), max_converge=1, nsweeps=1, with_matvec=False,
+                                       dmrg_form=False)
         print(json.dumps({"sweep_profile": out}), flush=True)
         return
 

@@ -197,13 +197,18 @@ int qlg_matvec(qlg_plan plan, const void *psi, void *out);
  * lexicographic order, prefix-sum offsets) is read with
  * qlg_plan_vector_layout; result axes are [Inverse(A.b), W.w1, A.b] for 'l'
  * and [Inverse(B.a), W.w0, B.a] for 'r'.
- * qlg_execute runs such a plan: in0/in1 = (A, B) or (env, mps); NULL reuses
- * the data pointers given at planning time; out is a caller buffer of the
- * layout's size.                                                           */
+ * qlg_plan_transpose: the tensor with its axes in a new order, result axis
+ * i = source axis perm[i] -- qlten's Transpose(axes) as the reference calls
+ * it (two_site_update_finite_tdvp_impl.h:36); the DMRG form uses it to lay
+ * block-operator groups out one operator after another.
+ * qlg_execute runs such a plan: in0/in1 = (A, B), (env, mps) or (t, unused);
+ * NULL reuses the data pointers given at planning time; out is a caller
+ * buffer of the layout's size.                                             */
 int qlg_plan_contract(qlg_ctx ctx, const qlg_bsten *A, const qlg_bsten *B, int k, int conj_a, int conj_b,
                       qlg_plan *out);
 int qlg_plan_env(qlg_ctx ctx, const qlg_bsten *env, const qlg_bsten *mps, const qlg_bsten *mpo, char side,
                  qlg_plan *out);
+int qlg_plan_transpose(qlg_ctx ctx, const qlg_bsten *t, const int32_t *perm, qlg_plan *out);
 int qlg_execute(qlg_plan plan, const void *in0, const void *in1, void *out);
 
 /* ---- Krylov vector kernels (flat vectors of n elements) ------------------

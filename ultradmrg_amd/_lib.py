@@ -30,7 +30,7 @@ EXPORTS = [
     "qlg_plan_two_site", "qlg_plan_one_site", "qlg_plan_super_block", "qlg_plan_destroy", "qlg_plan_set_precision",
     "qlg_plan_flops", "qlg_plan_vector_layout", "qlg_plan_workspace", "qlg_plan_enable_timing",
     "qlg_plan_stage_ms", "qlg_plan_stage_flops", "qlg_plan_stage_work", "qlg_matvec", "qlg_plan_contract", "qlg_plan_env",
-    "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
+    "qlg_plan_transpose", "qlg_execute", "qlg_dot", "qlg_nrm2", "qlg_scal", "qlg_axpy",
     "qlg_block_svd", "qlg_svd_result", "qlg_svd_bond", "qlg_svd_tensor", "qlg_svd_destroy",
     "qlg_linear_combine", "qlg_linear_combine_noalloc", "qlg_lanczos_gs", "qlg_lanczos_expmv", "qlg_eigh_batched", "qlg_plan_batch", "qlg_plan_batch_offsets",
     "qlg_plan_use_graphs", "qlg_plan_set_reorth", "qlg_lanczos_gs_batch", "qlg_tridiag_gs",
@@ -112,6 +112,7 @@ def lib():
                                         C.POINTER(vp)]),
         "qlg_plan_env": (C.c_int, [vp, C.POINTER(Bsten), C.POINTER(Bsten), C.POINTER(Bsten), C.c_char,
                                    C.POINTER(vp)]),
+        "qlg_plan_transpose": (C.c_int, [vp, C.POINTER(Bsten), C.POINTER(C.c_int32), C.POINTER(vp)]),
         "qlg_execute": (C.c_int, [vp, vp, vp, vp]),
         "qlg_dot": (C.c_int, [vp, C.c_int, i64, vp, vp, dp]),
         "qlg_nrm2": (C.c_int, [vp, C.c_int, i64, vp, dp]),

@@ -382,6 +382,16 @@ int qlg_plan_env(qlg_ctx ctx, const qlg_bsten *env, const qlg_bsten *mps, const 
   });
 }
 
+int qlg_plan_transpose(qlg_ctx ctx, const qlg_bsten *t, const int32_t *perm, qlg_plan *out) {
+  if (!ctx || !out) return QLG_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    auto h = std::make_unique<qlg_plan_s>();
+    h->p.reset(build_transpose_plan(&ctx->c, t, perm));
+    *out = h.release();
+  });
+}
+
 int qlg_execute(qlg_plan plan, const void *in0, const void *in1, void *out) {
   if (!plan || !out) return QLG_ERR_INVALID;
   Plan &p = *plan->p;

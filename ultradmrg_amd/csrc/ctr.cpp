@@ -10,6 +10,8 @@
 //    (UpdateSiteLenvs, include/qlmps/algorithm/vmps/vmps_init.h:359-373)
 //  * right environment R'[a*, w0, a] = B*[a', s', b'] R[b', w, b] W[w0, s, s', w] B[a, s, b]
 //    (UpdateSiteRenvs, vmps_init.h:343-357)
+//  * transpose: the axes of a tensor in a new order, result axis i = source
+//    axis perm[i] (qlten's Transpose, two_site_update_finite_tdvp_impl.h:36)
 //
 // Environment updates: stage 1 contracts the environment with the MPS tensor
 // into per-channel chunks, stage 2 applies the MPO tensor as a small dense
@@ -20,6 +22,7 @@
 #include <set>
 
 #include "plan.h"
+#include "transpose.h"
 
 namespace qlg {
 
@@ -543,12 +546,75 @@ Plan *build_env_plan(Ctx *ctx, const qlg_bsten *envd, const qlg_bsten *mpsd, con
   return P.release();
 }
 
-// Execute a tensordot / environment plan: in0, in1 are (A, B) for tensordot and
+// ---------------------------------------------------------------------------
+Plan *build_transpose_plan(Ctx *ctx, const qlg_bsten *td, const int32_t *perm) {
+  QLG_CHECK(td && perm, "transpose: null argument");
+  std::unique_ptr<Plan> P(new Plan());
+  Plan &p = *P;
+  p.ctx = ctx;
+  p.kind = PLAN_TRANSPOSE;
+  p.L = parse_tensor(td, td->rank, "t");
+  const Ten &t = p.L;
+  p.dtype = t.dtype;
+  const int r = t.rank;
+  std::vector<int> seen(r, 0);
+  for (int i = 0; i < r; ++i) {
+    QLG_CHECK(perm[i] >= 0 && perm[i] < r && !seen[perm[i]], "transpose: perm is not a permutation of the axes");
+    seen[perm[i]] = 1;
+  }
+  p.vec.rank = r;
+  for (int i = 0; i < r; ++i) p.vec.ax.push_back(t.ax[perm[i]]);
+  std::set<Coords> outs;
+  std::vector<Coords> dst_of(t.blk.size());
+  for (size_t b = 0; b < t.blk.size(); ++b) {
+    Coords c{};
+    for (int i = 0; i < r; ++i) c[i] = t.blk[b][perm[i]];
+    QLG_CHECK(outs.insert(c).second, "transpose: duplicate block");
+    dst_of[b] = c;
+  }
+  finish_layout(p.vec, outs);
+  std::vector<PermJob> jobs;
+  for (size_t b = 0; b < t.blk.size(); ++b) {
+    PermJob j{};
+    j.src = t.off[b];
+    j.dst = p.vec.off[p.vec.find[dst_of[b]]];
+    j.n = t.blk_elems(b);
+    j.rank = r;
+    // destination strides of destination axes, then mapped to source axes
+    int64_t ds[QLG_MAX_RANK], acc = 1;
+    for (int i = r - 1; i >= 0; --i) {
+      ds[i] = acc;
+      acc *= p.vec.ax[i].deg[dst_of[b][i]];
+    }
+    for (int i = 0; i < r; ++i) {
+      j.dim[i] = t.ax[i].deg[t.blk[b][i]];
+      j.dstride[perm[i]] = ds[i];
+    }
+    if (j.n == 0) continue;
+    p.perm_maxel = std::max(p.perm_maxel, j.n);
+    jobs.push_back(j);
+  }
+  p.perm_n = (int64_t)jobs.size();
+  p.perm_jobs.ensure(std::max<size_t>(1, jobs.size()) * sizeof(PermJob));
+  if (!jobs.empty())
+    QLG_HIP(hipMemcpyAsync(p.perm_jobs.p, jobs.data(), jobs.size() * sizeof(PermJob), hipMemcpyHostToDevice,
+                           ctx->stream));
+  QLG_HIP(hipStreamSynchronize(ctx->stream));
+  p.nstages = 0;
+  return P.release();
+}
+
+// Execute a tensordot / environment / transpose plan: in0, in1 are (A, B) for tensordot and
 // (env, mps) for environment updates; null pointers use the plan-time data.
 void Plan::execute(const void *in0, const void *in1, void *out) {
   Slots s{};
   auto dp = [](const void *p) { return const_cast<double *>(static_cast<const double *>(p)); };
   s.p[SLOT_OUT] = static_cast<double *>(out);
+  if (kind == PLAN_TRANSPOSE) {
+    launch_transpose(ctx->stream, perm_jobs.as<PermJob>(), perm_n, perm_maxel,
+                     static_cast<const double *>(in0 ? in0 : L.data), static_cast<double *>(out), elem_doubles(dtype));
+    return;
+  }
   if (kind == PLAN_TENSORDOT) {
     s.p[SLOT_L] = dp(in0 ? in0 : L.data);
     s.p[SLOT_R] = dp(in1 ? in1 : R.data);

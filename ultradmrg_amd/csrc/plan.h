@@ -45,7 +45,7 @@ struct Layout {
   }
 };
 
-enum PlanKind { PLAN_HEFF = 0, PLAN_TENSORDOT = 1, PLAN_ENV_LEFT = 2, PLAN_ENV_RIGHT = 3 };
+enum PlanKind { PLAN_HEFF = 0, PLAN_TENSORDOT = 1, PLAN_ENV_LEFT = 2, PLAN_ENV_RIGHT = 3, PLAN_TRANSPOSE = 4 };
 
 struct Plan {
   Ctx *ctx = nullptr;
@@ -117,6 +117,8 @@ struct Plan {
   double *host_seg = nullptr;    // pinned per-chain scalars (2 x chains)
   size_t host_seg_n = 0;
   std::vector<std::unique_ptr<DevBuf>> owned;   // operators the plan built itself (super-block form)
+  DevBuf perm_jobs;              // transpose plans: one PermJob per block
+  int64_t perm_n = 0, perm_maxel = 0;
   ~Plan();
   void matvec(const double *in, double *out);        // canonical in / out
   void matvec_fused(const double *in, double *out);  // fused in / out
@@ -127,6 +129,7 @@ struct Plan {
 
 Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *L, const qlg_bsten *const *W, const qlg_bsten *R,
                  const qlg_bsten *psi, int precision);
+Plan *build_transpose_plan(Ctx *ctx, const qlg_bsten *t, const int32_t *perm);
 Plan *build_super_block_plan(Ctx *ctx, int64_t ngroups, const int64_t *nleft, const qlg_term *left,
                              const int64_t *nright, const qlg_term *right, const qlg_bsten *psi, int precision);
 Plan *build_batch_plan(Ctx *ctx, int64_t n, Plan *const *members);

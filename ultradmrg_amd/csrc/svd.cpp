@@ -277,8 +277,10 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
                            s.tr ? 1 : 0, 0});
     }
     upload(c, dtab, tab);
-    svd_gather(stream, dtab.as<SvdBlk>(), (int64_t)tab.size(), static_cast<const double *>(t.data), X.as<double>(),
-               ed);
+    int64_t mx = 0;
+    for (const SvdBlk &b : tab) mx = std::max(mx, b.rows * b.cols);
+    svd_gather(stream, dtab.as<SvdBlk>(), (int64_t)tab.size(), mx, static_cast<const double *>(t.data),
+               X.as<double>(), ed);
   }
   std::vector<SvdMat> xm;   // per sector (bucket order): its X [n x m]
   for (const Bucket &B : bk)
@@ -286,9 +288,19 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       const Sector &s = sec[B.secs[j]];
       xm.push_back(SvdMat{B.xbase + (int64_t)j * B.b * B.mp, s.n, s.m, B.mp, 0});
     }
+  int64_t xmax = 0, rowmax = 0;   // largest sector matrix, most rows
+  for (const SvdMat &m : xm) {
+    xmax = std::max(xmax, m.rows * m.cols);
+    rowmax = std::max(rowmax, m.rows);
+  }
   scale.ensure(std::max<size_t>(1, xm.size()) * 8);
   upload(c, dmats, xm);
-  svd_normalise(stream, dmats.as<SvdMat>(), (int64_t)xm.size(), X.as<double>(), scale.as<double>(), ed);
+  {
+    DevBuf part;
+    part.ensure(std::max<size_t>(1, xm.size() * (size_t)svd_grid_y(xmax)) * 8);
+    svd_normalise(stream, dmats.as<SvdMat>(), (int64_t)xm.size(), xmax, X.as<double>(), part.as<double>(),
+                  scale.as<double>(), ed);
+  }
   {
     GemmBatch gb(c, cplx);
     for (const Bucket &B : bk)
@@ -422,7 +434,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     }
   sv.ensure((size_t)std::max<int64_t>(nsv, 1) * 8);
   upload(c, dmats, rm);
-  svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), R.as<double>(), sv.as<double>(), ed);
+  svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
   std::vector<double> hsv, hscale;
   download<double>(c, hscale, scale.p, xm.size());
   download<double>(c, hsv, sv.p, (size_t)nsv);
@@ -488,7 +500,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       DevBuf dj;
       upload(c, dj, jobs);
       svd_jacobi_rows(stream, dj.as<SvdJac>(), (int64_t)jobs.size(), R.as<double>(), G.as<double>(), ed);
-      svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), R.as<double>(), sv.as<double>(), ed);
+      svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
       download<double>(c, hsv, sv.p, (size_t)nsv);
       truncate(scaled_values(), keep, err, D);
     }
@@ -553,10 +565,13 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
   upload(c, dmask, mask);
   upload(c, dpj, pjobs);
   upload(c, dej, ejobs);
-  svd_gather_rows(stream, dpj.as<SvdRows>(), (int64_t)pjobs.size(), dperm.as<int32_t>(), dinv.as<double>(),
+  int64_t pmax = 0, emax = 0;
+  for (const SvdRows &j : pjobs) pmax = std::max(pmax, j.k * j.cols);
+  for (const SvdRows &j : ejobs) emax = std::max(emax, j.k * j.cols);
+  svd_gather_rows(stream, dpj.as<SvdRows>(), (int64_t)pjobs.size(), pmax, dperm.as<int32_t>(), dinv.as<double>(),
                   R.as<double>(), P.as<double>(), ed);
-  svd_gather_rows(stream, dej.as<SvdRows>(), (int64_t)ejobs.size(), dperm.as<int32_t>(), nullptr, G.as<double>(),
-                  EK.as<double>(), ed);
+  svd_gather_rows(stream, dej.as<SvdRows>(), (int64_t)ejobs.size(), emax, dperm.as<int32_t>(), nullptr,
+                  G.as<double>(), EK.as<double>(), ed);
 
   // ---- Newton-Schulz on the kept rows of P --------------------------------------------------
   {
@@ -583,11 +598,16 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       }
       g1.run();
       upload(c, dam, am);
-      dev.ensure(std::max<size_t>(1, am.size()) * 8);
-      svd_ns_dev(stream, dam.as<SvdMat>(), (int64_t)am.size(), A.as<double>(), dmask.as<double>(), dev.as<double>(),
-                 ed);
-      std::vector<double> hd;
-      download<double>(c, hd, dev.p, am.size());
+      int64_t amax = 0;
+      for (const SvdMat &m : am) amax = std::max(amax, m.rows * m.rows);
+      const int gy = svd_grid_y(amax);
+      dev.ensure(std::max<size_t>(1, am.size() * (size_t)gy) * 8);
+      svd_ns_dev(stream, dam.as<SvdMat>(), (int64_t)am.size(), amax, A.as<double>(), dmask.as<double>(),
+                 dev.as<double>(), ed);
+      std::vector<double> hdp, hd(am.size(), 0.0);
+      download<double>(c, hdp, dev.p, am.size() * (size_t)gy);
+      for (size_t q = 0; q < am.size(); ++q)
+        for (int y = 0; y < gy; ++y) hd[q] = std::max(hd[q], hdp[q * gy + y]);
       if (debug)
         for (size_t q = 0; q < active.size(); ++q)
           std::fprintf(stderr, "[qlg svd] NS round %d sector q=(%d,%d) k=%lld dev %.3e\n", round,
@@ -610,7 +630,9 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       }
       g2.run();
       upload(c, dam, um);
-      svd_ns_update(stream, dam.as<SvdMat>(), (int64_t)um.size(), P.as<double>(), T.as<double>(), ed);
+      int64_t umax = 0;
+      for (const SvdMat &m : um) umax = std::max(umax, m.rows * m.cols * ed);
+      svd_ns_update(stream, dam.as<SvdMat>(), (int64_t)um.size(), umax, P.as<double>(), T.as<double>(), ed);
       active.swap(still);
     }
     // rows of exactly zero singular values: completed to an orthonormal set
@@ -702,10 +724,17 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     upload(c, d1, ue);
     upload(c, d2, vp);
     upload(c, d3, ve);
-    svd_assemble(stream, d0.as<SvdOut>(), (int64_t)up.size(), P.as<double>(), res->data[0].as<double>(), ed);
-    svd_assemble(stream, d1.as<SvdOut>(), (int64_t)ue.size(), EK.as<double>(), res->data[0].as<double>(), ed);
-    svd_assemble(stream, d2.as<SvdOut>(), (int64_t)vp.size(), P.as<double>(), res->data[2].as<double>(), ed);
-    svd_assemble(stream, d3.as<SvdOut>(), (int64_t)ve.size(), EK.as<double>(), res->data[2].as<double>(), ed);
+    auto most = [](const std::vector<SvdOut> &v) {
+      int64_t m = 0;
+      for (const SvdOut &o : v) m = std::max(m, o.rows * o.cols);
+      return m;
+    };
+    svd_assemble(stream, d0.as<SvdOut>(), (int64_t)up.size(), most(up), P.as<double>(), res->data[0].as<double>(), ed);
+    svd_assemble(stream, d1.as<SvdOut>(), (int64_t)ue.size(), most(ue), EK.as<double>(), res->data[0].as<double>(),
+                 ed);
+    svd_assemble(stream, d2.as<SvdOut>(), (int64_t)vp.size(), most(vp), P.as<double>(), res->data[2].as<double>(), ed);
+    svd_assemble(stream, d3.as<SvdOut>(), (int64_t)ve.size(), most(ve), EK.as<double>(), res->data[2].as<double>(),
+                 ed);
     QLG_HIP(hipStreamSynchronize(stream));
   }
   // s: diagonal blocks (mi, mi), real
@@ -726,7 +755,9 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     DevBuf dsv, dd;
     upload(c, dsv, kept_sv);
     upload(c, dd, dj);
-    svd_diag(stream, dd.as<SvdRows>(), (int64_t)dj.size(), dsv.as<double>(), res->data[1].as<double>());
+    int64_t kmax = 0;
+    for (const SvdRows &j : dj) kmax = std::max(kmax, j.k * j.k);
+    svd_diag(stream, dd.as<SvdRows>(), (int64_t)dj.size(), kmax, dsv.as<double>(), res->data[1].as<double>());
     QLG_HIP(hipStreamSynchronize(stream));
   }
   return guard.release();

@@ -33,20 +33,28 @@ struct SvdJac {
   int64_t roff, rld, m, eoff, eld, n;
 };
 void svd_jacobi_rows(hipStream_t st, const SvdJac *jobs, int64_t n, double *r, double *e, int ed);
-void svd_gather(hipStream_t st, const SvdBlk *tab, int64_t n, const double *src, double *dst, int ed);
-void svd_normalise(hipStream_t st, const SvdMat *mats, int64_t n, double *x, double *scale, int ed);
+// Per-matrix kernels run on (matrices x gy) workgroups; gy = svd_grid_y(largest
+// element count of the launch's matrices).  maxel / maxrows: that largest count.
+int svd_grid_y(int64_t maxel);
+void svd_gather(hipStream_t st, const SvdBlk *tab, int64_t n, int64_t maxel, const double *src, double *dst, int ed);
+// part: n * svd_grid_y(maxel) doubles of scratch
+void svd_normalise(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, double *x, double *part,
+                   double *scale, int ed);
 void svd_pad_diag(hipStream_t st, const SvdMat *mats, int64_t n, double *g, int ed);
 void svd_reflect(hipStream_t st, const SvdMat *mats, int64_t n, int64_t bmax, double *g, const double *u);
 void svd_reflect_rows(hipStream_t st, const SvdMat *mats, int64_t n, double *e, const double *u);
-void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, const double *x, double *out, int ed);
-void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, const int32_t *perm, const double *f,
-                     const double *src, double *dst, int ed);
-void svd_ns_dev(hipStream_t st, const SvdMat *mats, int64_t n, const double *a, const double *mask, double *dev,
-                int ed);
-void svd_ns_update(hipStream_t st, const SvdMat *mats, int64_t n, double *p, const double *t, int ed);
+void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxrows, const double *x, double *out,
+                   int ed);
+void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
+                     const double *f, const double *src, double *dst, int ed);
+// dev: n * svd_grid_y(maxel) values, the deviation of matrix i is the max of dev[i * gy .. i * gy + gy)
+void svd_ns_dev(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, const double *a, const double *mask,
+                double *dev, int ed);
+void svd_ns_update(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, double *p, const double *t, int ed);
 void svd_complete_rows(hipStream_t st, const SvdMat *mats, int64_t n, double *p, double *mask, int ed);
-void svd_assemble(hipStream_t st, const SvdOut *jobs, int64_t n, const double *src, double *dst, int ed);
-void svd_diag(hipStream_t st, const SvdRows *jobs, int64_t n, const double *sv, double *dst);
+void svd_assemble(hipStream_t st, const SvdOut *jobs, int64_t n, int64_t maxel, const double *src, double *dst,
+                  int ed);
+void svd_diag(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const double *sv, double *dst);
 
 // the result of qlg_block_svd: the new bond and the three tensors' layouts and data
 struct SvdResult {

@@ -6,6 +6,8 @@
 // in doubles (ed = 1 real, 2 complex).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "svd.h"
 
 namespace qlg {
@@ -44,14 +46,20 @@ __device__ inline double block_max(double v) {
   return s;
 }
 
-// one workgroup per block: element (r, c) of the block's [rows x cols] row-major
-// data goes to row rr = r0 + r, column cc = c0 + c of its sector matrix, stored
-// transposed (short side first) when tr is set
+// Every per-matrix kernel below runs on a grid (matrices, gy): workgroup
+// (x, y) handles elements y * VT + t, (y + gy) * VT + t, ... of matrix x, so a
+// large matrix is spread over gy workgroups (gy from svd_grid_y).
+__device__ inline int64_t first_el() { return (int64_t)blockIdx.y * VT + threadIdx.x; }
+__device__ inline int64_t el_stride() { return (int64_t)gridDim.y * VT; }
+
+// element (r, c) of a block's [rows x cols] row-major data goes to row
+// rr = r0 + r, column cc = c0 + c of its sector matrix, stored transposed
+// (short side first) when tr is set
 __global__ __launch_bounds__(VT) void gather_kernel(const SvdBlk *__restrict__ tab, const double *__restrict__ src,
                                                     double *__restrict__ dst, int ed) {
   const SvdBlk b = tab[blockIdx.x];
   const int64_t tot = b.rows * b.cols;
-  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
     const int64_t r = e / b.cols, c = e % b.cols;
     const int64_t rr = b.r0 + r, cc = b.c0 + c;
     const int64_t pos = b.base + (b.tr ? cc * b.ld + rr : rr * b.ld + cc);
@@ -59,24 +67,37 @@ __global__ __launch_bounds__(VT) void gather_kernel(const SvdBlk *__restrict__ t
   }
 }
 
-// scale[i] = Frobenius norm of matrix i (rows x cols within leading dimension ld), then
-// the matrix is divided by it (norm 0: left as is, scale 1)
-__global__ __launch_bounds__(VT) void normalise_kernel(const SvdMat *__restrict__ mats, double *__restrict__ x,
-                                                       double *__restrict__ scale, int ed) {
+// Frobenius norms in two passes: each workgroup's sum of squares of its share
+// of matrix i (rows x cols within leading dimension ld) goes to
+// part[i * gy + y]; then every workgroup adds matrix i's gy partial sums in
+// the same order, divides its share by the norm (norm 0: left as is) and
+// workgroup (i, 0) writes scale[i] (1 for norm 0)
+__global__ __launch_bounds__(VT) void norm_partial_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ x,
+                                                          double *__restrict__ part, int ed) {
   const SvdMat m = mats[blockIdx.x];
   double s = 0.0;
   const int64_t tot = m.rows * m.cols;
-  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
     const int64_t p = m.off + (e / m.cols) * m.ld + e % m.cols;
     for (int q = 0; q < ed; ++q) s += x[p * ed + q] * x[p * ed + q];
   }
-  const double nrm = sqrt(block_sum(s));
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.x * gridDim.y + blockIdx.y] = s;
+}
+__global__ __launch_bounds__(VT) void normalise_kernel(const SvdMat *__restrict__ mats, double *__restrict__ x,
+                                                       const double *__restrict__ part, double *__restrict__ scale,
+                                                       int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  double s = 0.0;
+  for (unsigned y = 0; y < gridDim.y; ++y) s += part[(int64_t)blockIdx.x * gridDim.y + y];
+  const double nrm = sqrt(s);
   const double inv = nrm > 0.0 ? 1.0 / nrm : 1.0;
-  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+  const int64_t tot = m.rows * m.cols;
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
     const int64_t p = m.off + (e / m.cols) * m.ld + e % m.cols;
     for (int q = 0; q < ed; ++q) x[p * ed + q] *= inv;
   }
-  if (threadIdx.x == 0) scale[blockIdx.x] = nrm > 0.0 ? nrm : 1.0;
+  if (threadIdx.x == 0 && blockIdx.y == 0) scale[blockIdx.x] = nrm > 0.0 ? nrm : 1.0;
 }
 
 // Gram matrix padding: diagonal entries j >= n of the b x b matrix get
@@ -149,12 +170,12 @@ __global__ __launch_bounds__(VT) void reflect_rows_kernel(const SvdMat *__restri
   }
 }
 
-// norms of rows: one wave per row of each matrix; out[m.aux + r]
+// norms of rows: one wave per row (rows y * 4 + wave, stepping by 4 gy); out[m.aux + r]
 __global__ __launch_bounds__(VT) void row_norms_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ x,
                                                        double *__restrict__ out, int ed) {
   const SvdMat m = mats[blockIdx.x];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int64_t r = wave; r < m.rows; r += VT / 64) {
+  for (int64_t r = (int64_t)blockIdx.y * (VT / 64) + wave; r < m.rows; r += (int64_t)gridDim.y * (VT / 64)) {
     double s = 0.0;
     for (int64_t c = lane; c < m.cols; c += 64) {
       const int64_t p = m.off + r * m.ld + c;
@@ -171,7 +192,7 @@ __global__ __launch_bounds__(VT) void gather_rows_kernel(const SvdRows *__restri
                                                          double *__restrict__ dst, int ed) {
   const SvdRows j = jobs[blockIdx.x];
   const int64_t tot = j.k * j.cols;
-  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
     const int64_t i = e / j.cols, c = e % j.cols;
     const int64_t sr = perm[j.perm + i];
     const double fi = f ? f[j.perm + i] : 1.0;
@@ -180,13 +201,14 @@ __global__ __launch_bounds__(VT) void gather_rows_kernel(const SvdRows *__restri
   }
 }
 
-// Newton-Schulz deviation: max over (i, j) of |A_ij - mask_i delta_ij| for each k x k matrix
+// Newton-Schulz deviation: max over (i, j) of |A_ij - mask_i delta_ij| for each
+// k x k matrix, per workgroup share: dev[i * gy + y] (the caller takes the max over y)
 __global__ __launch_bounds__(VT) void ns_dev_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ a,
                                                     const double *__restrict__ mask, double *__restrict__ dev, int ed) {
   const SvdMat m = mats[blockIdx.x];
   double mx = 0.0;
   const int64_t k = m.rows;
-  for (int64_t e = threadIdx.x; e < k * k; e += VT) {
+  for (int64_t e = first_el(); e < k * k; e += el_stride()) {
     const int64_t i = e / k, j = e % k;
     const int64_t p = m.off + i * m.ld + j;
     double re = a[p * ed], im = ed == 2 ? a[p * ed + 1] : 0.0;
@@ -194,7 +216,7 @@ __global__ __launch_bounds__(VT) void ns_dev_kernel(const SvdMat *__restrict__ m
     mx = fmax(mx, sqrt(re * re + im * im));
   }
   mx = block_max(mx);
-  if (threadIdx.x == 0) dev[blockIdx.x] = mx;
+  if (threadIdx.x == 0) dev[(int64_t)blockIdx.x * gridDim.y + blockIdx.y] = mx;
 }
 
 // p <- 1.5 p - 0.5 t over each listed k x cols matrix (contiguous rows)
@@ -202,7 +224,7 @@ __global__ __launch_bounds__(VT) void ns_update_kernel(const SvdMat *__restrict_
                                                        const double *__restrict__ t, int ed) {
   const SvdMat m = mats[blockIdx.x];
   const int64_t tot = m.rows * m.cols * ed;
-  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
     const int64_t i = m.off * ed + e;
     p[i] = 1.5 * p[i] - 0.5 * t[m.aux * ed + e];
   }
@@ -352,12 +374,12 @@ __global__ __launch_bounds__(1024) void jacobi_rows_kernel(const SvdJac *__restr
 }
 
 // output assembly: dst[o + r * cols + c] = src value at base + r * rs + c * cs
-// (conjugated if conj), one workgroup per output block
+// (conjugated if conj), one row of the grid per output block
 __global__ __launch_bounds__(VT) void assemble_kernel(const SvdOut *__restrict__ jobs, const double *__restrict__ src,
                                                       double *__restrict__ dst, int ed) {
   const SvdOut j = jobs[blockIdx.x];
   const int64_t tot = j.rows * j.cols;
-  for (int64_t e = threadIdx.x; e < tot; e += VT) {
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
     const int64_t r = e / j.cols, c = e % j.cols;
     const int64_t s = j.base + r * j.rs + c * j.cs;
     dst[(j.dst + e) * ed] = src[s * ed];
@@ -369,7 +391,7 @@ __global__ __launch_bounds__(VT) void assemble_kernel(const SvdOut *__restrict__
 __global__ __launch_bounds__(VT) void diag_kernel(const SvdRows *__restrict__ jobs, const double *__restrict__ sv,
                                                   double *__restrict__ dst) {
   const SvdRows j = jobs[blockIdx.x];
-  for (int64_t e = threadIdx.x; e < j.k * j.k; e += VT) {
+  for (int64_t e = first_el(); e < j.k * j.k; e += el_stride()) {
     const int64_t r = e / j.k, c = e % j.k;
     dst[j.dst + e] = r == c ? sv[j.src + r] : 0.0;
   }
@@ -383,12 +405,28 @@ __global__ __launch_bounds__(VT) void diag_kernel(const SvdRows *__restrict__ jo
       QLG_HIP(hipGetLastError());                                                       \
     }                                                                                   \
   } while (0)
+#define QLG_LAUNCH2(kern, n, gy, ...)                                                           \
+  do {                                                                                          \
+    if ((n) > 0) {                                                                              \
+      hipLaunchKernelGGL(kern, dim3((unsigned)(n), (unsigned)(gy)), dim3(VT), 0, st, __VA_ARGS__); \
+      QLG_HIP(hipGetLastError());                                                               \
+    }                                                                                           \
+  } while (0)
 
-void svd_gather(hipStream_t st, const SvdBlk *tab, int64_t n, const double *src, double *dst, int ed) {
-  QLG_LAUNCH(gather_kernel, n, tab, src, dst, ed);
+int svd_grid_y(int64_t maxel) {
+  // about 16 elements per thread, at most 256 workgroups per matrix
+  const int64_t gy = (maxel + 16 * VT - 1) / (16 * VT);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(gy, 256));
 }
-void svd_normalise(hipStream_t st, const SvdMat *mats, int64_t n, double *x, double *scale, int ed) {
-  QLG_LAUNCH(normalise_kernel, n, mats, x, scale, ed);
+
+void svd_gather(hipStream_t st, const SvdBlk *tab, int64_t n, int64_t maxel, const double *src, double *dst, int ed) {
+  QLG_LAUNCH2(gather_kernel, n, svd_grid_y(maxel), tab, src, dst, ed);
+}
+void svd_normalise(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, double *x, double *part,
+                   double *scale, int ed) {
+  const int gy = svd_grid_y(maxel);
+  QLG_LAUNCH2(norm_partial_kernel, n, gy, mats, x, part, ed);
+  QLG_LAUNCH2(normalise_kernel, n, gy, mats, x, part, scale, ed);
 }
 void svd_pad_diag(hipStream_t st, const SvdMat *mats, int64_t n, double *g, int ed) {
   QLG_LAUNCH(pad_diag_kernel, n, mats, g, ed);
@@ -402,28 +440,31 @@ void svd_reflect(hipStream_t st, const SvdMat *mats, int64_t n, int64_t bmax, do
 void svd_reflect_rows(hipStream_t st, const SvdMat *mats, int64_t n, double *e, const double *u) {
   QLG_LAUNCH(reflect_rows_kernel, n, mats, reinterpret_cast<double2 *>(e), reinterpret_cast<const double2 *>(u));
 }
-void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, const double *x, double *out, int ed) {
-  QLG_LAUNCH(row_norms_kernel, n, mats, x, out, ed);
+void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxrows, const double *x, double *out,
+                   int ed) {
+  const int64_t gy = std::max<int64_t>(1, std::min<int64_t>((maxrows + VT / 64 - 1) / (VT / 64), 1024));
+  QLG_LAUNCH2(row_norms_kernel, n, gy, mats, x, out, ed);
 }
-void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, const int32_t *perm, const double *f,
-                     const double *src, double *dst, int ed) {
-  QLG_LAUNCH(gather_rows_kernel, n, jobs, perm, f, src, dst, ed);
+void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
+                     const double *f, const double *src, double *dst, int ed) {
+  QLG_LAUNCH2(gather_rows_kernel, n, svd_grid_y(maxel), jobs, perm, f, src, dst, ed);
 }
-void svd_ns_dev(hipStream_t st, const SvdMat *mats, int64_t n, const double *a, const double *mask, double *dev,
-                int ed) {
-  QLG_LAUNCH(ns_dev_kernel, n, mats, a, mask, dev, ed);
+void svd_ns_dev(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, const double *a, const double *mask,
+                double *dev, int ed) {
+  QLG_LAUNCH2(ns_dev_kernel, n, svd_grid_y(maxel), mats, a, mask, dev, ed);
 }
-void svd_ns_update(hipStream_t st, const SvdMat *mats, int64_t n, double *p, const double *t, int ed) {
-  QLG_LAUNCH(ns_update_kernel, n, mats, p, t, ed);
+void svd_ns_update(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, double *p, const double *t, int ed) {
+  QLG_LAUNCH2(ns_update_kernel, n, svd_grid_y(maxel), mats, p, t, ed);
 }
 void svd_complete_rows(hipStream_t st, const SvdMat *mats, int64_t n, double *p, double *mask, int ed) {
   QLG_LAUNCH(complete_rows_kernel, n, mats, p, mask, ed);
 }
-void svd_assemble(hipStream_t st, const SvdOut *jobs, int64_t n, const double *src, double *dst, int ed) {
-  QLG_LAUNCH(assemble_kernel, n, jobs, src, dst, ed);
+void svd_assemble(hipStream_t st, const SvdOut *jobs, int64_t n, int64_t maxel, const double *src, double *dst,
+                  int ed) {
+  QLG_LAUNCH2(assemble_kernel, n, svd_grid_y(maxel), jobs, src, dst, ed);
 }
-void svd_diag(hipStream_t st, const SvdRows *jobs, int64_t n, const double *sv, double *dst) {
-  QLG_LAUNCH(diag_kernel, n, jobs, sv, dst);
+void svd_diag(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const double *sv, double *dst) {
+  QLG_LAUNCH2(diag_kernel, n, svd_grid_y(maxel), jobs, sv, dst);
 }
 void svd_jacobi_rows(hipStream_t st, const SvdJac *jobs, int64_t n, double *r, double *e, int ed) {
   if (n <= 0) return;

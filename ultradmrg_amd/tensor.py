@@ -287,6 +287,20 @@ def contract(A: BlockTensor, B: BlockTensor, k: int = 1, conj_a=False, conj_b=Fa
     return plan.run(A, B, dtype)
 
 
+def transpose(t: BlockTensor, perm: Sequence[int], ctx: Context = None) -> BlockTensor:
+    """The tensor with axis i of the result = axis perm[i] of t (qlten's
+    Transpose, as at two_site_update_finite_tdvp_impl.h:36), on the GPU."""
+    ctx = ctx or default_context(t.data.device.index or 0)
+    perm = [int(p) for p in perm]
+    h = C.c_void_p()
+    d = t.descriptor()
+    pa = (C.c_int32 * len(perm))(*perm)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.qlg_plan_transpose(ctx.h, C.byref(d), pa, C.byref(h)))
+    plan = _OpPlan(ctx, h, [t.axes[p] for p in perm], t.rank)
+    return plan.run(t, t, t.data.dtype)
+
+
 def env_update(env: BlockTensor, mps: BlockTensor, mpo: BlockTensor, side: str,
                ctx: Context = None) -> BlockTensor:
     """UpdateSiteLenvs (side 'l') / UpdateSiteRenvs (side 'r') on the GPU."""
