@@ -250,6 +250,8 @@ def heisenberg_ramp_measurement(L=100, ramp=(256, 384, 512, 768, 1024, 1024), se
         drv.p = SweepParams(1, Dr, Dr, 1e-10, lp, verbose=False)
         energies.append(drv.sweep())
         secs.append(drv.stats.sweep_seconds[-1])
+        print(f"[bench] Heisenberg L={L} sweep D={Dr}: E = {energies[-1]:.10f} ({secs[-1]:.1f} s, "
+              f"SVD {drv.stats.svd_s:.1f} s so far)", file=sys.stderr, flush=True)
     total_s = time.perf_counter() - t0
     mid = L // 2 - 1
     for i in range(mid):

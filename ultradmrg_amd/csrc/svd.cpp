@@ -483,23 +483,64 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     // (eps s_max^2 / (s_i s_j) relative) that Newton-Schulz cannot untangle.
     // One-sided Jacobi on the sector's rows of R, with the same rotations on
     // its rows of E^H (X = E R is kept exactly), resolves them; the refined
-    // values then decide the truncation.
+    // values then decide the truncation.  Refined values move the pooled cut,
+    // which can make another sector keep values below the threshold: the check
+    // repeats until every such sector is refined (each at most once).
     constexpr double kTau = 1e-6;
-    std::vector<SvdJac> jobs;
-    for (size_t si = 0; si < sec.size(); ++si) {
-      const Sector &s = sec[si];
-      if (keep[si] == 0 || s.n < 2) continue;
-      std::vector<double> v(hsv.begin() + s.sv0, hsv.begin() + s.sv0 + s.n);
-      std::sort(v.begin(), v.end(), std::greater<double>());
-      if (v[keep[si] - 1] >= kTau * v[0]) continue;
-      const Bucket &B = bk[s.bucket];
-      jobs.push_back(SvdJac{B.rbase + (int64_t)s.slot * B.b * B.mp, B.mp, s.m,
-                            B.gbase + (int64_t)s.slot * B.b * B.b + (B.b - s.n) * B.b, B.b, s.n});
-    }
-    if (!jobs.empty()) {
-      DevBuf dj;
+    std::vector<char> refined(sec.size(), 0);
+    for (int pass = 0; pass < 8; ++pass) {
+      // jobs: the sector's rows above its rounding floor (64 eps of its largest
+      // value; rows below it are reported as zero and completed anyway),
+      // largest first
+      std::vector<SvdJacB> jobs;
+      std::vector<int32_t> idx, pairs;
+      int maxnbp = 0;
+      for (size_t si = 0; si < sec.size(); ++si) {
+        const Sector &s = sec[si];
+        if (refined[si] || keep[si] == 0 || s.n < 2) continue;
+        std::vector<int32_t> ord(s.n);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int32_t a, int32_t b) { return hsv[s.sv0 + a] > hsv[s.sv0 + b]; });
+        const double vmax = hsv[s.sv0 + ord[0]];
+        if (hsv[s.sv0 + ord[keep[si] - 1]] >= kTau * vmax) continue;
+        refined[si] = 1;
+        const double floor = 64.0 * 2.220446049250313e-16 * vmax;
+        int64_t k = 0;
+        while (k < s.n && hsv[s.sv0 + ord[k]] > floor) ++k;
+        if (k < 2) continue;
+        const Bucket &B = bk[s.bucket];
+        const int nblk = (int)((k + 15) / 16), nbp = nblk + (nblk & 1);
+        const int job = (int)jobs.size();
+        jobs.push_back(SvdJacB{B.rbase + (int64_t)s.slot * B.b * B.mp, B.mp, s.m,
+                               B.gbase + (int64_t)s.slot * B.b * B.b + (B.b - s.n) * B.b, B.b, k,
+                               (int64_t)idx.size(), nblk, 0});
+        idx.insert(idx.end(), ord.begin(), ord.begin() + k);
+        for (int pi = 0; pi < nbp / 2; ++pi) {
+          pairs.push_back(job);
+          pairs.push_back(pi);
+        }
+        maxnbp = std::max(maxnbp, nbp);
+      }
+      if (jobs.empty()) break;
+      DevBuf dj, dpairs, didx, dflag;
       upload(c, dj, jobs);
-      svd_jacobi_rows(stream, dj.as<SvdJac>(), (int64_t)jobs.size(), R.as<double>(), G.as<double>(), ed);
+      upload(c, dpairs, pairs);
+      upload(c, didx, idx);
+      dflag.ensure(jobs.size() * sizeof(int32_t));
+      std::vector<int32_t> hflag;
+      for (int sweep = 0; sweep < 30; ++sweep) {
+        QLG_HIP(hipMemsetAsync(dflag.p, 0, jobs.size() * sizeof(int32_t), stream));
+        for (int round = 0; round < maxnbp - 1; ++round)
+          svd_jacobi_round(stream, dj.as<SvdJacB>(), dpairs.as<int32_t>(), (int64_t)pairs.size() / 2,
+                           didx.as<int32_t>(), round, R.as<double>(), G.as<double>(), dflag.as<int32_t>(), ed);
+        download<int32_t>(c, hflag, dflag.p, jobs.size());
+        bool any = false;
+        for (int32_t f : hflag) any |= f != 0;
+        if (debug) std::fprintf(stderr, "[qlg svd] Jacobi pass %d sweep %d: %zu jobs, rotated %d\n", pass, sweep,
+                                jobs.size(), (int)any);
+        if (!any) break;
+      }
       svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
       download<double>(c, hsv, sv.p, (size_t)nsv);
       truncate(scaled_values(), keep, err, D);
@@ -617,8 +658,20 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       std::vector<SvdMat> um;
       for (size_t q = 0; q < active.size(); ++q) {
         if (hd[q] < 1e-13) continue;
-        if (!(hd[q] <= 0.5))
-          throw Error(QLG_ERR_NUMERIC, "block SVD: singular vectors too far from orthonormal");
+        if (!(hd[q] <= 0.5)) {
+          const Sector &bad = sec[kept[active[q]]];
+          double vmin = 1e300, vmax = 0.0;
+          for (int64_t i = 0; i < bad.n; ++i) vmax = std::max(vmax, hsv[bad.sv0 + i]);
+          for (int64_t i = 0; i < keep[kept[active[q]]]; ++i)
+            vmin = std::min(vmin, hsv[bad.sv0 + perm[kbase[kept[active[q]]] + i]]);
+          char msg[320];
+          std::snprintf(msg, sizeof msg,
+                        "block SVD: singular vectors too far from orthonormal (sector (%d,%d), %lld x %lld, "
+                        "%lld kept, smallest kept / largest value %.3e, deviation %.3e after %d steps)",
+                        bad.q[0], bad.q[1], (long long)bad.R, (long long)bad.C,
+                        (long long)keep[kept[active[q]]], vmin / vmax, hd[q], round);
+          throw Error(QLG_ERR_NUMERIC, msg);
+        }
         const int i = active[q];
         const Sector &s = sec[kept[i]];
         const int64_t k = keep[kept[i]];

@@ -27,12 +27,17 @@ struct SvdOut {
   int32_t conj, pad;
 };
 
-// one-sided Jacobi job: n rows of R (length m, leading dimension rld) at roff,
-// and the same n rows of E^H (length and leading dimension eld) at eoff
-struct SvdJac {
-  int64_t roff, rld, m, eoff, eld, n;
+// blocked one-sided Jacobi job: k rows of R (length m, leading dimension rld)
+// at roff, listed by index at idx[idx .. idx + k), and the same rows of E^H
+// (length and leading dimension eld) at eoff; nblk = ceil(k / 16) blocks
+struct SvdJacB {
+  int64_t roff, rld, m, eoff, eld, k, idx;
+  int32_t nblk, pad;
 };
-void svd_jacobi_rows(hipStream_t st, const SvdJac *jobs, int64_t n, double *r, double *e, int ed);
+// one round of a sweep: pairs = (job, player) for every block pair slot of
+// every job; flag[job] is set when a pair of that job needed a rotation
+void svd_jacobi_round(hipStream_t st, const SvdJacB *jobs, const int32_t *pairs, int64_t npairs, const int32_t *idx,
+                      int round, double *r, double *e, int32_t *flag, int ed);
 // Per-matrix kernels run on (matrices x gy) workgroups; gy = svd_grid_y(largest
 // element count of the launch's matrices).  maxel / maxrows: that largest count.
 int svd_grid_y(int64_t maxel);

@@ -290,86 +290,234 @@ __global__ __launch_bounds__(VT) void complete_rows_kernel(const SvdMat *__restr
   }
 }
 
-// One-sided (Hestenes) Jacobi on the n rows of R (length m) of one sector per
-// workgroup, with every rotation applied to the same two rows of E^H (length
-// b) as well, so X = E R is kept exactly: rows i, j with alpha = |r_i|^2,
-// beta = |r_j|^2, gamma = <r_i, r_j> = |gamma| e^{i phi} are replaced by
-//   r_i <- c r_i - s e^{i phi} r_j,  r_j <- s r_i + c e^{i phi} r_j
-// with t = s / c the smaller root of t^2 + 2 zeta t - 1 = 0,
-// zeta = (beta - alpha) / (2 |gamma|), which makes them orthogonal.  Pairs run
-// in round-robin order (all disjoint pairs of a round in parallel, one wave
-// each); sweeps repeat until no pair has |gamma| > 1e-15 sqrt(alpha beta)
-// (at most 40).  Afterwards the row norms are the singular values.
-__global__ __launch_bounds__(1024) void jacobi_rows_kernel(const SvdJac *__restrict__ jobs, double *__restrict__ r,
-                                                           double *__restrict__ e, int ed) {
-  const SvdJac j = jobs[blockIdx.x];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  const int64_t n = j.n, np = n + (n & 1);   // players, with a dummy when n is odd
-  __shared__ int rotated;
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    if (threadIdx.x == 0) rotated = 0;
+// Blocked one-sided (Hestenes) Jacobi on rows of R, with every row
+// transformation applied to the same rows of E^H as well, so X = E R is kept
+// exactly.  A job's rows (idx[job.idx .. + k), largest values first) are cut
+// into blocks of JB = 16; a round pairs the blocks round-robin and one
+// workgroup takes one block pair (32 rows):
+//   1. g = R_pq R_pq^H, the 32 x 32 Gram of the pair's rows, accumulated over
+//      the row length in LDS-staged chunks;
+//   2. two-sided cyclic Jacobi on g in LDS: the rotation of rows (i, j) with
+//      alpha = g_ii, beta = g_jj, gamma = g_ij = |gamma| e^{i phi} is
+//        row_i <- c row_i - s e^{i phi} row_j,  row_j <- s row_i + c e^{i phi} row_j,
+//      t = s / c the smaller root of t^2 + 2 zeta t - 1 = 0,
+//      zeta = (beta - alpha) / (2 |gamma|); g <- T g T^H and the pair
+//      transformation T <- T_ij T, until no pair has |gamma| > tol sqrt(alpha beta)
+//      (at most 8 inner sweeps), tol = 8 eps sqrt(m): an inner product of two
+//      length-m rows is known to about eps sqrt(m) of |r_i| |r_j|;
+//   3. the pair's rows of R and of E^H <- T (rows), chunk by chunk.
+// A pair that needed a rotation in its first inner sweep flags its job as
+// not yet converged (flag[job] = 1); the host repeats sweeps until no flag is
+// set.  Relative accuracy: each rotation is decided on |gamma| relative to
+// sqrt(alpha beta), as in the scalar method, so small rows are resolved
+// against large ones to ~eps of their own norm.
+constexpr int JB = 16, JP = 2 * JB, JC = 32;   // block rows, pair rows, staged columns
+
+template <bool CPLX>
+struct JElem;
+template <>
+struct JElem<false> {
+  using T = double;
+};
+template <>
+struct JElem<true> {
+  using T = double2;
+};
+
+__device__ inline double jre(double x) { return x; }
+__device__ inline double jre(double2 x) { return x.x; }
+__device__ inline double jabs2(double x) { return x * x; }
+__device__ inline double jabs2(double2 x) { return x.x * x.x + x.y * x.y; }
+__device__ inline double jmul_conj(double a, double b) { return a * b; }
+__device__ inline double2 jmul_conj(double2 a, double2 b) {   // a conj(b)
+  return double2{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+__device__ inline double jmul(double a, double b) { return a * b; }
+__device__ inline double2 jmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline double jadd(double a, double b) { return a + b; }
+__device__ inline double2 jadd(double2 a, double2 b) { return double2{a.x + b.x, a.y + b.y}; }
+__device__ inline double jscale(double a, double f) { return a * f; }
+__device__ inline double2 jscale(double2 a, double f) { return double2{a.x * f, a.y * f}; }
+__device__ inline double jconj(double a) { return a; }
+__device__ inline double2 jconj(double2 a) { return double2{a.x, -a.y}; }
+template <class T>
+__device__ inline T jzero();
+template <>
+__device__ inline double jzero<double>() { return 0.0; }
+template <>
+__device__ inline double2 jzero<double2>() { return double2{0.0, 0.0}; }
+template <class T>
+__device__ inline T jone();
+template <>
+__device__ inline double jone<double>() { return 1.0; }
+template <>
+__device__ inline double2 jone<double2>() { return double2{1.0, 0.0}; }
+template <class T>
+__device__ inline T jphase(T g, double ag);
+template <>
+__device__ inline double jphase<double>(double g, double ag) { return g / ag; }
+template <>
+__device__ inline double2 jphase<double2>(double2 g, double ag) { return double2{g.x / ag, g.y / ag}; }
+
+template <bool CPLX>
+__global__ __launch_bounds__(VT) void jacobi_round_kernel(const SvdJacB *__restrict__ jobs,
+                                                          const int32_t *__restrict__ pairs,
+                                                          const int32_t *__restrict__ idx, int round,
+                                                          typename JElem<CPLX>::T *__restrict__ r,
+                                                          typename JElem<CPLX>::T *__restrict__ e,
+                                                          int32_t *__restrict__ flag) {
+  using T = typename JElem<CPLX>::T;
+  const int job = pairs[2 * blockIdx.x], pi = pairs[2 * blockIdx.x + 1];
+  const SvdJacB j = jobs[job];
+  const int nbp = j.nblk + (j.nblk & 1);
+  if (round >= nbp - 1) return;   // this job's sweep is shorter
+  int a, b;
+  if (pi == 0) {
+    a = nbp - 1;
+    b = round;
+  } else {
+    a = (round + pi) % (nbp - 1);
+    b = (round - pi + nbp - 1) % (nbp - 1);
+  }
+  // a block paired with the dummy block (odd block count) is processed alone:
+  // the dummy's rows are absent (rowid -1), its own pairs still rotate
+  __shared__ T g[JP][JP + 1], tr[JP][JP + 1], st[JP][JC + 1];
+  __shared__ int rowid[JP];
+  __shared__ double rc[JP / 2], rs[JP / 2];
+  __shared__ T rph[JP / 2];
+  __shared__ int rp[JP / 2], rq[JP / 2];
+  __shared__ int any_rot, any_first;
+  const int tid = threadIdx.x;
+  if (tid < JP) {
+    const int blk = tid < JB ? a : b, slot = blk * JB + (tid % JB);
+    rowid[tid] = slot < j.k ? idx[j.idx + slot] : -1;
+  }
+  if (tid == 0) any_first = 0;
+  __syncthreads();
+  const double tol = fmax(1e-15, 8.0 * 2.220446049250313e-16 * sqrt((double)j.m));
+  // ---- 1. Gram of the pair's rows ------------------------------------------------
+  T acc[JP * JP / VT];
+#pragma unroll
+  for (int u = 0; u < JP * JP / VT; ++u) acc[u] = jzero<T>();
+  for (int64_t c0 = 0; c0 < j.m; c0 += JC) {
+    for (int q = tid; q < JP * JC; q += VT) {
+      const int i = q / JC, c = q % JC;
+      const int row = rowid[i];
+      st[i][c] = (row >= 0 && c0 + c < j.m) ? r[j.roff + (int64_t)row * j.rld + c0 + c] : jzero<T>();
+    }
     __syncthreads();
-    for (int64_t rd = 0; rd < np - 1; ++rd) {
-      for (int64_t pi = wave; pi < np / 2; pi += nw) {
-        int64_t a, b;
-        if (pi == 0) {
-          a = np - 1;
-          b = rd;
+#pragma unroll
+    for (int u = 0; u < JP * JP / VT; ++u) {
+      const int q = tid + u * VT, i = q / JP, k = q % JP;
+      T s = acc[u];
+      for (int c = 0; c < JC; ++c) s = jadd(s, jmul_conj(st[i][c], st[k][c]));
+      acc[u] = s;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < JP * JP / VT; ++u) {
+    const int q = tid + u * VT, i = q / JP, k = q % JP;
+    g[i][k] = acc[u];
+    tr[i][k] = i == k ? jone<T>() : jzero<T>();
+  }
+  __syncthreads();
+  // exact Hermitian symmetry: the lower triangle mirrors the upper one
+  for (int q = tid; q < JP * JP; q += VT) {
+    const int i = q / JP, k = q % JP;
+    if (i > k) g[i][k] = jconj(g[k][i]);
+  }
+  __syncthreads();
+  // ---- 2. cyclic Jacobi on g, accumulating T -------------------------------------------
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    if (tid == 0) any_rot = 0;
+    __syncthreads();
+    for (int step = 0; step < JP - 1; ++step) {
+      if (tid < JP / 2) {
+        int p, q;
+        if (tid == 0) {
+          p = JP - 1;
+          q = step;
         } else {
-          a = (rd + pi) % (np - 1);
-          b = (rd - pi + np - 1) % (np - 1);
+          p = (step + tid) % (JP - 1);
+          q = (step - tid + JP - 1) % (JP - 1);
         }
-        if (a >= n || b >= n) continue;
-        double *x = r + (j.roff + a * j.rld) * ed, *y = r + (j.roff + b * j.rld) * ed;
-        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
-        for (int64_t c = lane; c < j.m; c += 64) {
-          if (ed == 2) {
-            const double xr = x[2 * c], xi = x[2 * c + 1], yr = y[2 * c], yi = y[2 * c + 1];
-            al += xr * xr + xi * xi;
-            be += yr * yr + yi * yi;
-            gr += xr * yr + xi * yi;   // x conj(y)
-            gi += xi * yr - xr * yi;
-          } else {
-            al += x[c] * x[c];
-            be += y[c] * y[c];
-            gr += x[c] * y[c];
-          }
+        const double al = jre(g[p][p]), be = jre(g[q][q]);
+        const T gam = g[p][q];
+        const double ag = sqrt(jabs2(gam));
+        double cs = 1.0, sn = 0.0;
+        T ph = jone<T>();
+        if (al > 0.0 && be > 0.0 && ag > tol * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * ag);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          cs = 1.0 / sqrt(1.0 + t * t);
+          sn = cs * t;
+          ph = jphase<T>(gam, ag);
+          any_rot = 1;
         }
-        al = wave_sum(al);
-        be = wave_sum(be);
-        gr = wave_sum(gr);
-        gi = wave_sum(gi);
-        const double g = sqrt(gr * gr + gi * gi);
-        if (!(g > 1e-15 * sqrt(al * be)) || al == 0.0 || be == 0.0) continue;
-        const double zeta = (be - al) / (2.0 * g);
-        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-        const double pr = gr / g, pim = gi / g;   // e^{i phi}
-        if (lane == 0) rotated = 1;
-        auto rot = [&](double *u, double *v, int64_t len) {
-          for (int64_t c = lane; c < len; c += 64) {
-            if (ed == 2) {
-              const double ur = u[2 * c], ui = u[2 * c + 1], vr = v[2 * c], vi = v[2 * c + 1];
-              const double wr = pr * vr - pim * vi, wi = pr * vi + pim * vr;   // e^{i phi} v
-              u[2 * c] = cs * ur - sn * wr;
-              u[2 * c + 1] = cs * ui - sn * wi;
-              v[2 * c] = sn * ur + cs * wr;
-              v[2 * c + 1] = sn * ui + cs * wi;
-            } else {
-              const double uu = u[c], vv = pr * v[c];
-              u[c] = cs * uu - sn * vv;
-              v[c] = sn * uu + cs * vv;
-            }
-          }
-        };
-        rot(x, y, j.m);
-        rot(e + (j.eoff + a * j.eld) * ed, e + (j.eoff + b * j.eld) * ed, j.eld);
+        rp[tid] = p;
+        rq[tid] = q;
+        rc[tid] = cs;
+        rs[tid] = sn;
+        rph[tid] = ph;
+      }
+      __syncthreads();
+      // rows of g and T: row_p <- c row_p - s ph row_q, row_q <- s row_p + c ph row_q
+      for (int w = tid; w < (JP / 2) * JP; w += VT) {
+        const int pr = w / JP, col = w % JP;
+        if (rs[pr] == 0.0) continue;
+        const int p = rp[pr], q = rq[pr];
+        const double cs = rc[pr], sn = rs[pr];
+        const T ph = rph[pr];
+        const T gp = g[p][col], gq = jmul(ph, g[q][col]);
+        g[p][col] = jadd(jscale(gp, cs), jscale(gq, -sn));
+        g[q][col] = jadd(jscale(gp, sn), jscale(gq, cs));
+        const T tp = tr[p][col], tq = jmul(ph, tr[q][col]);
+        tr[p][col] = jadd(jscale(tp, cs), jscale(tq, -sn));
+        tr[q][col] = jadd(jscale(tp, sn), jscale(tq, cs));
+      }
+      __syncthreads();
+      // columns of g: col_p <- c col_p - s conj(ph) col_q, col_q <- s col_p + c conj(ph) col_q
+      for (int w = tid; w < (JP / 2) * JP; w += VT) {
+        const int pr = w / JP, row = w % JP;
+        if (rs[pr] == 0.0) continue;
+        const int p = rp[pr], q = rq[pr];
+        const double cs = rc[pr], sn = rs[pr];
+        const T ph = jconj(rph[pr]);
+        const T gp = g[row][p], gq = jmul(ph, g[row][q]);
+        g[row][p] = jadd(jscale(gp, cs), jscale(gq, -sn));
+        g[row][q] = jadd(jscale(gp, sn), jscale(gq, cs));
       }
       __syncthreads();
     }
-    const int any = rotated;
+    const int rot = any_rot;
+    if (sweep == 0 && tid == 0) any_first = rot;
     __syncthreads();
-    if (!any) break;
+    if (!rot) break;
+  }
+  if (!any_first) return;   // the pair was already orthogonal: rows unchanged
+  if (tid == 0) flag[job] = 1;
+  // ---- 3. rows of R and E^H <- T rows --------------------------------------------------
+  for (int which = 0; which < 2; ++which) {
+    T *base = which == 0 ? r + j.roff : e + j.eoff;
+    const int64_t ld = which == 0 ? j.rld : j.eld, len = which == 0 ? j.m : j.eld;
+    for (int64_t c0 = 0; c0 < len; c0 += JC) {
+      for (int q = tid; q < JP * JC; q += VT) {
+        const int i = q / JC, c = q % JC;
+        const int row = rowid[i];
+        st[i][c] = (row >= 0 && c0 + c < len) ? base[(int64_t)row * ld + c0 + c] : jzero<T>();
+      }
+      __syncthreads();
+      for (int q = tid; q < JP * JC; q += VT) {
+        const int i = q / JC, c = q % JC;
+        const int row = rowid[i];
+        if (row < 0 || c0 + c >= len) continue;
+        T s = jzero<T>();
+        for (int l = 0; l < JP; ++l) s = jadd(s, jmul(tr[i][l], st[l][c]));
+        base[(int64_t)row * ld + c0 + c] = s;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -466,9 +614,15 @@ void svd_assemble(hipStream_t st, const SvdOut *jobs, int64_t n, int64_t maxel, 
 void svd_diag(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const double *sv, double *dst) {
   QLG_LAUNCH2(diag_kernel, n, svd_grid_y(maxel), jobs, sv, dst);
 }
-void svd_jacobi_rows(hipStream_t st, const SvdJac *jobs, int64_t n, double *r, double *e, int ed) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(jacobi_rows_kernel, dim3((unsigned)n), dim3(1024), 0, st, jobs, r, e, ed);
+void svd_jacobi_round(hipStream_t st, const SvdJacB *jobs, const int32_t *pairs, int64_t npairs, const int32_t *idx,
+                      int round, double *r, double *e, int32_t *flag, int ed) {
+  if (npairs <= 0) return;
+  if (ed == 2)
+    hipLaunchKernelGGL(jacobi_round_kernel<true>, dim3((unsigned)npairs), dim3(VT), 0, st, jobs, pairs, idx, round,
+                       reinterpret_cast<double2 *>(r), reinterpret_cast<double2 *>(e), flag);
+  else
+    hipLaunchKernelGGL(jacobi_round_kernel<false>, dim3((unsigned)npairs), dim3(VT), 0, st, jobs, pairs, idx, round, r,
+                       e, flag);
   QLG_HIP(hipGetLastError());
 }
 
