@@ -100,3 +100,41 @@ def test_block_svd_rank4_two_site_shape():
     rs = sorted(np.concatenate([np.diag(v) for v in ref.s.blocks.values()]))[::-1]
     gs = sorted(np.concatenate([np.diag(v) for v in out.s.blocks().values()]))[::-1]
     assert np.abs(np.array(rs) - np.array(gs)).max() <= 1e-12 * rs[0]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_block_svd_degenerate_gram(cplx):
+    """Sectors whose Gram matrix a unitary rotation leaves diagonal: exactly
+    orthonormal rows with the order a multiple of 16 (Gram = I, no padding),
+    a scaled copy of them, and an all-zero sector, beside a generic one (the
+    complex eigensolver returns NaN vectors on such input; the library then
+    finishes those sectors by Jacobi rotations).  Values vs numpy, orthonormal
+    u and vt, reconstruction (no truncation) to 1e-10 of the largest entry:
+    the Newton-Schulz orthonormalisation moves u s vt by up to
+    ~eps s_max^2 / s_min, 2e-10 for the generic sector's 1e-6 spread."""
+    print(f"[case] library block svd degenerate Gram cplx={cplx}", flush=True)
+    rng = np.random.default_rng(5)
+    ph = np.exp(1j * 0.3) if cplx else 1.0
+    dims = [(16, 40), (32, 48), (16, 16), (24, 30)]
+    a_ax = Axis.make([((2 * i,), r) for i, (r, _) in enumerate(dims)], -1)
+    b_ax = Axis.make([((2 * i,), c) for i, (_, c) in enumerate(dims)], 1)
+    blocks = {}
+    blocks[(0, 0)] = np.hstack([np.eye(16), np.zeros((16, 24))]) * ph
+    blocks[(1, 1)] = 0.5 * np.hstack([np.zeros((32, 16)), np.eye(32)]) * ph
+    blocks[(2, 2)] = np.zeros((16, 16)) * ph
+    blocks[(3, 3)] = (_orth(rng, 24, 24, cplx) * np.logspace(0, -6, 24)) @ _orth(rng, 30, 24, cplx).conj().T
+    t = BlockTensor.from_blocks([a_ax, b_ax], blocks,
+                                dtype=torch.complex128 if cplx else torch.float64)
+    r = block_svd(t, 1, (0,), 0.0, 10 ** 6, 10 ** 6)
+    spectra = {i: np.linalg.svd(b, compute_uv=False) for (i, _), b in blocks.items()}
+    for (mi, _), blk in r.s.blocks().items():
+        i = r.u.axes[-1].qns[mi][0] // 2
+        got = np.real(np.diag(blk))
+        assert np.abs(got - spectra[i][:len(got)]).max() <= 1e-13
+    for (i, mi), U in r.u.blocks().items():
+        assert np.abs(U.conj().T @ U - np.eye(U.shape[1])).max() < 1e-12
+    for (mi, j), V in r.vt.blocks().items():
+        assert np.abs(V @ V.conj().T - np.eye(V.shape[0])).max() < 1e-12
+    rec = contract(contract(r.u, r.s, 1), r.vt, 1).blocks()
+    for k, b in blocks.items():
+        assert np.abs(rec.get(k, np.zeros_like(b)) - b).max() < 1e-10

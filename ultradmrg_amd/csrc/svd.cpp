@@ -320,6 +320,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
   }
 
   // ---- eigen-decompositions (rows of E^H = eigenvectors^H, ascending) ------------------
+  std::vector<char> forced(sec.size(), 0);   // sectors left to the Jacobi refinement (E = I)
   DevBuf W;
   W.ensure((size_t)std::max<int64_t>(gtot, 1) * 8);   // eigenvalues, b per matrix (an upper bound)
   {
@@ -330,7 +331,9 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       wt += (int64_t)bk[i].secs.size() * bk[i].b;
     }
     // complex: rotate by H2 H1, solve, rotate the vectors back; a batch with NaN
-    // vectors is solved again with other reflectors (three attempts)
+    // vectors is solved again with other reflectors; a matrix that still has
+    // NaN vectors (a Gram matrix the rotation leaves diagonal: c I, or zero)
+    // gets E = I and its sector is finished by the Jacobi refinement below
     std::vector<int> todo;
     for (size_t i = 0; i < bk.size(); ++i)
       if (bk[i].b > 1) todo.push_back((int)i);
@@ -339,7 +342,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       G0.ensure((size_t)std::max<int64_t>(gtot, 1) * ed * 8);
       QLG_HIP(hipMemcpyAsync(G0.p, G.p, (size_t)gtot * ed * 8, hipMemcpyDeviceToDevice, stream));
     }
-    for (int attempt = 0; attempt < (cplx ? 3 : 1) && !todo.empty(); ++attempt) {
+    for (int attempt = 0; attempt < (cplx ? 2 : 1) && !todo.empty(); ++attempt) {
       std::vector<SvdMat> gm;
       std::vector<double> uh;
       int64_t bmax = 0;
@@ -400,8 +403,24 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
         if (bad) left.push_back(i);
       }
       todo.swap(left);
-      if (!todo.empty() && attempt == 2)
-        throw Error(QLG_ERR_NUMERIC, "block SVD: the eigensolver returned NaN for complex Gram batches");
+    }
+    // matrices still with NaN vectors: E^H row (b - n + i) = e_i, so R = X
+    for (int i : todo) {
+      const Bucket &B = bk[i];
+      std::vector<double> h;
+      download<double>(c, h, G.as<double>() + B.gbase * ed, (size_t)B.secs.size() * B.b * B.b * ed);
+      for (size_t j = 0; j < B.secs.size(); ++j) {
+        double *m = h.data() + j * B.b * B.b * ed;
+        bool bad = false;
+        for (int64_t e = 0; e < B.b * B.b * ed && !bad; ++e) bad = std::isnan(m[e]);
+        if (!bad) continue;
+        const Sector &sj = sec[B.secs[j]];
+        std::fill(m, m + B.b * B.b * ed, 0.0);
+        for (int64_t r = 0; r < sj.n; ++r) m[((B.b - sj.n + r) * B.b + r) * ed] = 1.0;
+        forced[B.secs[j]] = 1;
+      }
+      QLG_HIP(hipMemcpyAsync(G.as<double>() + B.gbase * ed, h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
+      QLG_HIP(hipStreamSynchronize(stream));
     }
     for (Bucket &B : bk)   // the b = 1 bucket: E^H = [1]
       if (B.b == 1) {
@@ -497,13 +516,14 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       int maxnbp = 0;
       for (size_t si = 0; si < sec.size(); ++si) {
         const Sector &s = sec[si];
-        if (refined[si] || keep[si] == 0 || s.n < 2) continue;
+        if (refined[si] || s.n < 2) continue;
+        if (keep[si] == 0 && !forced[si]) continue;
         std::vector<int32_t> ord(s.n);
         std::iota(ord.begin(), ord.end(), 0);
         std::stable_sort(ord.begin(), ord.end(),
                          [&](int32_t a, int32_t b) { return hsv[s.sv0 + a] > hsv[s.sv0 + b]; });
         const double vmax = hsv[s.sv0 + ord[0]];
-        if (hsv[s.sv0 + ord[keep[si] - 1]] >= kTau * vmax) continue;
+        if (!forced[si] && hsv[s.sv0 + ord[keep[si] - 1]] >= kTau * vmax) continue;
         refined[si] = 1;
         const double floor = 64.0 * 2.220446049250313e-16 * vmax;
         int64_t k = 0;
