@@ -421,14 +421,17 @@ __global__ __launch_bounds__(256) void gemm_bf16_frag_kernel(const GemmTile *__r
 // Ap[buf][piece][m][.] and Bp[buf][piece][n][.] (32-deep slices: a row is
 // four 16-byte chunks of 8 consecutive k), and every MFMA fragment (8
 // consecutive k of one row) is one 16-byte read of a plane.  The chunks of a
-// row are stored XOR-swizzled by (row / 4) % 4, so the 16 rows one fragment
-// read touches land on distinct banks.  Threads stage K in quads of four
+// row are stored XOR-swizzled by ((row / 2) ^ (row / 4)) % 4: a row is 64
+// bytes, so any 8 consecutive rows of one fragment read land on distinct
+// 16-byte slots of a 128-byte bank window and any 16 on distinct slots of a
+// 256-byte one (the earlier (row / 4) % 4 left two-way conflicts in every
+// group of 8 lanes: 2.7-4 conflict cycles per LDS instruction counted).  Threads stage K in quads of four
 // consecutive values: A quads run along a row (coalesced); B quads run along
 // n when B is stored [K x N] and along k when it is stored [N x K].
 namespace {
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline int swz(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
+__device__ inline int swz(int row, int chunk) { return chunk ^ (((row >> 1) ^ (row >> 2)) & 3); }
 
 template <int NPIECE>
 __device__ inline void split4(const float (&x)[4], bf16x4 (&p)[NPIECE]) {
