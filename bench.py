@@ -17,8 +17,9 @@ HBM bandwidth with its algorithmic bytes.  ``traffic`` is the HBM bytes per
 launch of that kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters
 (scaled by a copy kernel of known size) and recorded in
 profiles/heff_traffic_hubbard_d2000.txt; it is null for other workloads.
-``cpu_baseline`` times the oracle's numpy matvec on this host over a bounded
-sample of the same workload.
+``cpu_baseline`` times the C++/OpenMP CPU restatement of the same matvec
+(block-sparse contractions with a host BLAS dgemm, oracle/heff_cpu.cpp) on this
+host's cores over a bounded sample of the same workload.
 
     python bench.py --copy-calibration   # 1 GiB device copy, for counter scaling
 
@@ -28,10 +29,10 @@ unless ``--no-sweep``, the sweep
 comparison of the headline metric (``sweep``): the 4x16 Hubbard cylinder
 (t=1, U=8, 56 electrons, Sz=0, the reference benchmark's initial product
 state, ``benchmark/ultradmrg_hubbard2d.cpp:130-248``) is ramped in FP64 through
-D = 10, 100, 500, 1000 (one sweep each) and three sweeps at D=2000, then two
-sweeps at D=2000 (Lanczos 1e-9 / 100, truncation 1e-8) are timed in FP64 and,
-from the same state, in the split-BF16 6-term and 3-term modes; energies are
-compared relative to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
+D = 10, 100, 1000 and then D=2000, up to 12 sweeps at each D (fewer once two
+consecutive sweep energies agree to 1e-8), then two sweeps at D=2000 (Lanczos
+1e-9 / 100, truncation 1e-8) are timed in FP64 and, from the same state, in the
+split-BF16 6-term and 3-term modes; energies are compared relative to FP64.  Unless ``--no-chains``, ``batching`` compares 32 independent
 random-field Heisenberg chains solved as one batch (one grouped launch per
 matvec stage, hipGraph-captured Lanczos iterations) with a loop over the
 chains; unless ``--no-tdvp``, ``tdvp`` times three complex128 TDVP steps at
@@ -46,6 +47,13 @@ import json
 import os
 import sys
 import time
+
+# The block SVD spreads its eigen-decompositions over four streams (each a
+# chain of small, latency-bound launches).  With HIP's default of four
+# hardware queues per process those streams share queues with the solver's
+# own streams and partly serialise; eight keep them apart (set before the
+# first HIP call; a value already in the environment wins).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)

@@ -22,6 +22,10 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <climits>
 #include <exception>
 #include <mutex>
@@ -176,6 +180,12 @@ void eigh_batched(Ctx &c, int dtype, int64_t ngroups, const qlg_eigh_group *grou
     work[j].push_back(k);
     load[j] += cost(k);
   }
+  // QLG_SVD_PROFILE=1: each lane drains its stream and the lanes' finishing
+  // times (from the fork) are printed
+  const char *prf = std::getenv("QLG_SVD_PROFILE");
+  const bool profile = prf && prf[0] == '1';
+  double lane_ms[kLanes] = {0, 0, 0, 0};
+  const auto t0 = std::chrono::steady_clock::now();
   QLG_HIP(hipEventRecord(p->fork, c.stream));
   std::exception_ptr err[kLanes];
   std::vector<std::thread> th;
@@ -188,12 +198,32 @@ void eigh_batched(Ctx &c, int dtype, int64_t ngroups, const qlg_eigh_group *grou
         QLG_HIP(hipStreamWaitEvent(l.stream, p->fork, 0));
         for (int64_t k : work[j]) solve_group(l.handle, dtype, groups[k], E + e_off[k], info + i_off[k]);
         QLG_HIP(hipEventRecord(l.done, l.stream));
+        if (profile) {
+          QLG_HIP(hipStreamSynchronize(l.stream));
+          lane_ms[j] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
       } catch (...) {
         err[j] = std::current_exception();
       }
     });
   }
   for (auto &t : th) t.join();
+  if (profile) {
+    std::string line;
+    char buf[96];
+    for (int j = 0; j < kLanes; ++j) {
+      if (work[j].empty()) continue;
+      std::snprintf(buf, sizeof buf, " lane %d [", j);
+      line += buf;
+      for (int64_t k : work[j]) {
+        std::snprintf(buf, sizeof buf, " %lldx%lld", (long long)groups[k].n, (long long)groups[k].batch);
+        line += buf;
+      }
+      std::snprintf(buf, sizeof buf, " ] %.2f ms", lane_ms[j]);
+      line += buf;
+    }
+    std::fprintf(stderr, "[qlg eigh prof]%s\n", line.c_str());
+  }
   for (int j = 0; j < kLanes; ++j)
     if (err[j]) {
       // work already queued on the other lanes writes the caller's buffers and

@@ -25,7 +25,8 @@ enum BaseSlot : uint8_t {
   SLOT_X = 5,     // first intermediate
   SLOT_Y = 6,     // second intermediate
   SLOT_C = 7,     // combined MPO coefficient matrices
-  NSLOTS = 8
+  SLOT_W = 8,     // split-K partial panels
+  NSLOTS = 9
 };
 
 // Segment flags: op(A) = A^T when A is stored [K x M] (row-major, lda);
@@ -79,6 +80,23 @@ struct TransDesc {
   int64_t sld, dsx;
   int32_t nz, nx;
 };
+
+// Split-K: a task whose K is long runs as `parts` tasks over consecutive K
+// ranges, each writing a dense partial panel [M x N] (ldc = N) at
+// wbase + part * M * N in the SLOT_W workspace; the reduce kernel sums the
+// panels in part order (deterministic) and writes the task's output exactly
+// as the task's own epilogue would (row table or row stride).
+struct RedTask {
+  int64_t c_off, ldc, wbase;
+  int32_t M, N, parts, c_row;
+  uint8_t c_slot, c_rowtab, pad[6];
+};
+
+// reduce launch unit: elements [e0, e0 + kRedChunk) of task `task`'s M x N panel
+struct RedChunk {
+  int32_t task, e0;
+};
+constexpr int kRedChunk = 1024;
 
 struct GemmTile {
   int32_t task;

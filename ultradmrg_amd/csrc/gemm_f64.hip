@@ -295,6 +295,52 @@ static void launch_shape(int bm, int bn, int bk, const GemmTile *tiles, const Ge
   throw Error(QLG_ERR_UNSUPPORTED, "unsupported FP64 tile shape");
 }
 
+// Split-K reduction: every thread sums up to four elements of one task's
+// partial panels in part order and writes them where the task's epilogue
+// would (row table or row stride).
+template <bool CPLX>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const RedTask *__restrict__ red,
+                                                            const RedChunk *__restrict__ chunks, Slots slots,
+                                                            int nchunks) {
+  if ((int)blockIdx.x >= nchunks) return;
+  if (slots.skip && *slots.skip) return;   // the Krylov solve has stopped
+  const RedChunk ch = chunks[blockIdx.x];
+  const RedTask t = red[ch.task];
+  const int64_t panel = (int64_t)t.M * t.N;
+  const double *W = slots.p[SLOT_W];
+  double *C = slots.p[t.c_slot];
+#pragma unroll
+  for (int j = 0; j < kRedChunk / 256; ++j) {
+    const int64_t e = (int64_t)ch.e0 + threadIdx.x + j * 256;
+    if (e >= panel || e >= (int64_t)ch.e0 + kRedChunk) break;
+    const int64_t r = e / t.N, c = e % t.N;
+    const int64_t ix = (t.c_rowtab ? slots.crows[t.c_row + r] - t.c_off : r * t.ldc) + c + t.c_off;
+    if constexpr (CPLX) {
+      double2 acc = reinterpret_cast<const double2 *>(W)[t.wbase + e];
+      for (int q = 1; q < t.parts; ++q) {
+        const double2 v = reinterpret_cast<const double2 *>(W)[t.wbase + q * panel + e];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      reinterpret_cast<double2 *>(C)[ix] = acc;
+    } else {
+      double acc = W[t.wbase + e];
+      for (int q = 1; q < t.parts; ++q) acc += W[t.wbase + q * panel + e];
+      C[ix] = acc;
+    }
+  }
+}
+
+void launch_splitk_reduce(bool cplx, const RedTask *red, const RedChunk *chunks, int nchunks, const Slots &slots,
+                          hipStream_t st) {
+  if (nchunks <= 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(splitk_reduce_kernel<true>, dim3(nchunks), dim3(256), 0, st, red, chunks, slots, nchunks);
+  else
+    hipLaunchKernelGGL(splitk_reduce_kernel<false>, dim3(nchunks), dim3(256), 0, st, red, chunks, slots, nchunks);
+  QLG_HIP(hipGetLastError());
+}
+
 int gemm_bk() {
   static int bk = [] {
     const char *v = std::getenv("QLG_BK");

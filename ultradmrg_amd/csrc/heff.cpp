@@ -289,6 +289,9 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     lst[a] = s;
   }
   Stage &s1 = p.st[0], &s2 = p.st[1], &s3 = p.st[2];
+  s3.splitk = true;   // one long-K task per bra sector: split-K parts may be tuned in
+  s3.split_on = std::getenv("QLG_SPLITK") != nullptr;
+  s3.ed = ed;
   std::map<int, int64_t> tbase;
   {
     int64_t t1 = 0;
@@ -767,6 +770,10 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       std::fprintf(stderr, "[qlg plan] stage %d tile efficiency %.3f; M p10/50/90 %d/%d/%d, N %d/%d/%d, K %d/%d/%d\n",
                    i + 1, padded > 0 ? useful / padded : 0.0, pct(ms, .1), pct(ms, .5), pct(ms, .9), pct(ns, .1),
                    pct(ns, .5), pct(ns, .9), pct(ks, .1), pct(ks, .5), pct(ks, .9));
+      if (s.split())
+        std::fprintf(stderr, "[qlg plan] stage %d split-K: chunk %lld, %zu tasks split into %zu launch tasks, %zu tiles, "
+                     "workspace %lld elements\n", i + 1, (long long)s.kchunk, s.red.size(), s.ktasks.size(),
+                     s.tiles.size(), (long long)s.work_elems);
     }
     std::fprintf(stderr, "[qlg plan] vector %lld, T1 %lld, Y %lld, C %lld, L stacks %lld, R panels %lld, algo gflop %.3f\n",
                  (long long)p.vec.size, (long long)p.x_elems, (long long)p.y_elems, (long long)p.c_elems,
@@ -871,6 +878,7 @@ Slots Plan::slots(const double *in, double *out) {
     s.ps[SLOT_R] = rint_elems;
     s.ps[SLOT_Y] = y_elems;
   }
+  s.p[SLOT_W] = st[2].work.as<double>();
   s.crows = st[2].d_crows.as<int64_t>();
   s.skip = skip_flag;
   return s;
@@ -887,24 +895,34 @@ void Plan::launch_stage(int i, const Slots &s) {
                      (int)g.tiles.size(), stream);
     return;
   }
+  // the parts write their panels through SLOT_W: it must name this stage's workspace as uploaded
+  QLG_CHECK(!g.split() || (s.p[SLOT_W] == g.work.p && g.work.bytes >= (size_t)g.work_elems * g.ed * sizeof(double)),
+            "split-K workspace does not match the stage's task list");
   if (out_planes) {
     launch_grouped_gemm_planar(precision, g.bm, g.bn, i == 2, g.d_tiles.as<GemmTile>(), g.d_tasks.as<GemmTask>(),
                                g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
-    return;
-  }
-  // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1;
-  // stage 3: A = Y, B = R panels (transposed), C = the Krylov output (FP64, row table)
-  if (reduced)
+  } else if (reduced) {
+    // stage 1: A = L stacks, B = the Krylov vector (FP64), C = T1;
+    // stage 3: A = Y, B = R panels (transposed), C = the Krylov output (FP64, row table)
     launch_grouped_gemm_mixed(precision, g.bm, g.bn, i == 0, i == 2, cplx, g.d_tiles.as<GemmTile>(),
                               g.d_tasks.as<GemmTask>(), g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
-  else
+  } else {
     launch_grouped_gemm_f64(g.bm, g.bn, cplx, g.d_tiles.as<GemmTile>(), g.d_tasks.as<GemmTask>(),
                             g.d_segs.as<GemmSeg>(), s, (int)g.tiles.size(), stream);
+  }
+  // split-K parts wrote FP64 partial panels: sum them into the output
+  if (g.split())
+    launch_splitk_reduce(cplx, g.d_red.as<RedTask>(), g.d_redc.as<RedChunk>(), (int)g.redc.size(), s, stream);
 }
 
 namespace {
 std::mutex g_tile_mu;
-std::unordered_map<std::string, std::pair<int, int>> g_tile_cache;   // signature -> (bm, bn)
+struct TileChoice {
+  int bm = 0, bn = 0;
+  bool split = false;   // split-K parts (stage 3)
+  bool operator==(const TileChoice &o) const { return bm == o.bm && bn == o.bn && split == o.split; }
+};
+std::unordered_map<std::string, TileChoice> g_tile_cache;   // signature -> tile shape and split-K
 int g_tune_hits = 0, g_tune_misses = 0;
 
 bool autotune_enabled() {
@@ -935,8 +953,8 @@ void Plan::autotune(const double *in, double *out) {
     const char *v = std::getenv("QLG_VERBOSE");
     return v && v[0] == '1';
   }();
-  // choices cached for this precision's shape signatures ({0, 0}: not tuned yet)
-  std::pair<int, int> cached[3] = {{0, 0}, {0, 0}, {0, 0}};
+  // choices cached for this precision's shape signatures (bm = 0: not tuned yet)
+  TileChoice cached[3];
   bool unchanged = step_graphs.empty() && chunk_graphs.empty();
   {
     std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -948,13 +966,15 @@ void Plan::autotune(const double *in, double *out) {
         cached[i] = it->second;
         ++g_tune_hits;
       }
-      if (cached[i] != std::make_pair(g.bm, g.bn) || g.tiles.empty()) unchanged = false;
+      if (!(cached[i] == TileChoice{g.bm, g.bn, g.split_on}) || g.tiles.empty()) unchanged = false;
     }
   }
   // both stages already hold the tiles chosen for this precision: nothing to
   // time or re-upload, and the stream need not drain
   if (unchanged) return;
-  const Slots s = slots(in, out);
+  // slot pointers are taken again after every upload: a tile list with
+  // split-K parts (re)allocates the stage's workspace, which SLOT_W names
+  Slots s = slots(in, out);
   // tile lists are re-uploaded (and their buffers may grow) below: nothing
   // queued earlier on the stream may still read them
   QLG_HIP(hipStreamSynchronize(stream));
@@ -965,18 +985,19 @@ void Plan::autotune(const double *in, double *out) {
     for (int i : {0, 2}) {
       Stage &g = st[i];
       if (g.tasks.empty()) continue;
-      std::pair<int, int> chosen = cached[i];
-      if (chosen.first == 0) {
+      TileChoice chosen = cached[i];
+      if (chosen.bm == 0) {
         if (i == 2) {   // fill T1 and Y with this matvec's data before stage 3 is timed
           launch_stage(0, s);
           launch_stage(1, s);
         }
         const auto rank = rank_tile_shapes(g, cplx, precision == QLG_PREC_F64 && !cplx);
         double best_ms = 1e300;
-        for (size_t c = 0; c < rank.size(); ++c) {
-          const auto sh = tile_shape(rank[c].second);
-          g.make_tiles(sh.first, sh.second);
+        auto time_choice = [&](const TileChoice &tc, double model_ms) {
+          g.split_on = tc.split;
+          g.make_tiles(tc.bm, tc.bn);
           g.upload(stream);
+          s = slots(in, out);
           double ms_min = 1e300;
           for (int rep = 0; rep < 3; ++rep) {   // one warm-up, best of two timed launches
             QLG_HIP(hipEventRecord(e0, stream));
@@ -988,20 +1009,37 @@ void Plan::autotune(const double *in, double *out) {
             if (rep > 0) ms_min = std::min(ms_min, (double)ms);
           }
           if (verbose)
-            std::fprintf(stderr, "[qlg tune] stage %d %dx%d: %.4f ms (model %.4f ms)\n", i + 1, sh.first, sh.second,
-                         ms_min, rank[c].first * 1e3);
+            std::fprintf(stderr, "[qlg tune] stage %d %dx%d: %.4f ms (model %.4f ms, split-K chunk %lld)\n", i + 1,
+                         tc.bm, tc.bn, ms_min, model_ms, (long long)g.kchunk);
           if (ms_min < best_ms) {
             best_ms = ms_min;
-            chosen = sh;
+            chosen = tc;
           }
+          return ms_min;
+        };
+        // every shape unsplit (or, with QLG_SPLITK set, split as it says);
+        // then the two fastest shapes with split-K parts
+        const bool forced = std::getenv("QLG_SPLITK") != nullptr;
+        std::vector<std::pair<double, TileChoice>> timed;
+        for (size_t c = 0; c < rank.size(); ++c) {
+          const auto sh = tile_shape(rank[c].second);
+          const TileChoice tc{sh.first, sh.second, forced && g.splitk};
+          timed.push_back({time_choice(tc, rank[c].first * 1e3), tc});
+        }
+        if (g.splitk && !forced) {
+          std::stable_sort(timed.begin(), timed.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+          for (size_t c = 0; c < std::min<size_t>(2, timed.size()); ++c)
+            time_choice(TileChoice{timed[c].second.bm, timed[c].second.bn, true}, 0.0);
         }
         std::lock_guard<std::mutex> lk(g_tile_mu);
         g_tile_cache[tile_signature(g, i, variant)] = chosen;
         ++g_tune_misses;
       }
-      if (chosen.first != g.bm || chosen.second != g.bn || g.tiles.empty()) {
-        g.make_tiles(chosen.first, chosen.second);
+      if (!(chosen == TileChoice{g.bm, g.bn, g.split_on}) || g.tiles.empty()) {
+        g.split_on = chosen.split;
+        g.make_tiles(chosen.bm, chosen.bn);
         g.upload(stream);
+        s = slots(in, out);
       }
     }
     drop_graphs();   // captured launches name the old tile lists
@@ -1014,8 +1052,8 @@ void Plan::autotune(const double *in, double *out) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (verbose)
-    std::fprintf(stderr, "[qlg tune] tiles %dx%d / %dx%d (cache hits %d, misses %d)\n", st[0].bm, st[0].bn, st[2].bm,
-                 st[2].bn, g_tune_hits, g_tune_misses);
+    std::fprintf(stderr, "[qlg tune] tiles %dx%d / %dx%d%s (cache hits %d, misses %d)\n", st[0].bm, st[0].bn,
+                 st[2].bm, st[2].bn, st[2].split() ? " split-K" : "", g_tune_hits, g_tune_misses);
   const char *vv = std::getenv("QLG_VERBOSE");
   if (vv && vv[0] == '3') {
     // task shapes of the two GEMM stages: FLOP-weighted quantiles of M, N, K

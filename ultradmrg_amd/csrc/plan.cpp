@@ -84,7 +84,129 @@ void Stage::make_tiles(int bm_, int bn_) {
       mnk.push_back({t.M, t.N, kp});
     }
   }
+  kchunk = 0;
+  ktasks.clear();
+  ksegs.clear();
+  red.clear();
+  redc.clear();
+  work_elems = 0;
+  if (splitk && split_on && chan.empty()) kchunk = pick_kchunk(mnk, bm, bn);
+  if (kchunk > 0) {
+    // Cut every task whose K exceeds 3/2 of the chunk into parts of at most
+    // kchunk.  A cut falls inside a segment only at a multiple of 64 of its
+    // K, so every part's operand offsets keep the alignment of the segment's
+    // own (16-byte rows of the bf16 planes).
+    std::vector<std::array<int64_t, 3>> kmnk;
+    for (size_t ti = 0; ti < tasks.size(); ++ti) {
+      const GemmTask &t = tasks[ti];
+      int64_t ktot = 0;
+      for (int q = 0; q < t.seg_count; ++q) ktot += segs[t.seg_begin + q].K;
+      if (ktot * 2 <= kchunk * 3 || t.M <= 0 || t.N <= 0) {
+        GemmTask c = t;
+        c.seg_begin = (int32_t)ksegs.size();
+        for (int q = 0; q < t.seg_count; ++q) ksegs.push_back(segs[t.seg_begin + q]);
+        ktasks.push_back(c);
+        kmnk.push_back(mnk[ti]);
+        continue;
+      }
+      RedTask r{};
+      r.c_off = t.c_off;
+      r.ldc = t.ldc;
+      r.c_row = t.c_row;
+      r.c_slot = t.c_slot;
+      r.c_rowtab = t.c_rowtab;
+      r.M = t.M;
+      r.N = t.N;
+      r.wbase = work_elems;
+      const int64_t panel = (int64_t)t.M * t.N;
+      int parts = 0;
+      GemmTask cur{};
+      int64_t have = 0;   // K in the part being filled
+      auto open_part = [&]() {
+        cur = GemmTask{};
+        cur.c_slot = SLOT_W;
+        cur.c_off = r.wbase + (int64_t)parts * panel;
+        cur.ldc = t.N;
+        cur.M = t.M;
+        cur.N = t.N;
+        cur.seg_begin = (int32_t)ksegs.size();
+        cur.seg_count = 0;
+        have = 0;
+      };
+      auto close_part = [&]() {
+        ktasks.push_back(cur);
+        kmnk.push_back({cur.M, cur.N, (have + 15) / 16 * 16});
+        ++parts;
+      };
+      open_part();
+      for (int q = 0; q < t.seg_count; ++q) {
+        const GemmSeg &g = segs[t.seg_begin + q];
+        const bool ta = g.flags & SEG_TRANS_A, tb = g.flags & SEG_TRANS_B;
+        int64_t k0 = 0;
+        while (k0 < g.K) {
+          if (have >= kchunk) {
+            close_part();
+            open_part();
+          }
+          int64_t take = std::min<int64_t>(g.K - k0, kchunk - have);
+          if (k0 + take < g.K) {
+            take = take / 64 * 64;   // cut at a multiple of 64 of the segment's K
+            if (take == 0) {
+              if (have == 0) take = std::min<int64_t>(g.K - k0, 64);   // kchunk >= 64: unreachable
+              else {
+                close_part();
+                open_part();
+                continue;
+              }
+            }
+          }
+          GemmSeg h = g;
+          h.K = (int32_t)take;
+          h.a_off = g.a_off + k0 * (ta ? g.lda : 1);
+          h.b_off = g.b_off + k0 * (tb ? 1 : g.ldb);
+          ksegs.push_back(h);
+          ++cur.seg_count;
+          have += take;
+          k0 += take;
+        }
+      }
+      if (cur.seg_count > 0) close_part();
+      r.parts = parts;
+      work_elems += (int64_t)parts * panel;
+      for (int64_t e = 0; e < panel; e += kRedChunk) redc.push_back(RedChunk{(int32_t)red.size(), (int32_t)e});
+      red.push_back(r);
+    }
+    QLG_CHECK(work_elems < (int64_t(1) << 31) / 2, "split-K workspace too large");
+    if (red.empty()) {
+      kchunk = 0;
+      ktasks.clear();
+      ksegs.clear();
+      make_tiles_lpt(tiles, mnk, bm, bn);
+      return;
+    }
+    make_tiles_lpt(tiles, kmnk, bm, bn);
+    return;
+  }
   make_tiles_lpt(tiles, mnk, bm, bn);
+}
+
+// Split-K chunk for a task list and tile shape.  A grouped launch ends with
+// its longest tiles: a tile's time grows with its K, and the stage-3 tasks
+// of the matvec (one per bra sector, one long segment each) put K up to a
+// few thousand on every tile of a task.  The chunk is the tile-K that one of
+// 2048 resident workgroups (256 CUs x 8) would process if the work were
+// spread evenly, halved, so the last tiles to start are short; at least 512
+// (shorter parts cost more in partial panels than they gain in balance).
+// QLG_SPLITK overrides it (0: off, else the chunk, rounded up to 64).
+int64_t pick_kchunk(const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn) {
+  if (const char *v = std::getenv("QLG_SPLITK")) {
+    const int64_t k = std::atoll(v);
+    return k <= 0 ? 0 : (k + 63) / 64 * 64;
+  }
+  double work = 0;
+  for (const auto &t : mnk) work += double((t[0] + bm - 1) / bm) * double((t[1] + bn - 1) / bn) * double(t[2]);
+  const int64_t k = (int64_t)(work / 2048.0 / 2.0);
+  return std::max<int64_t>(512, (k + 63) / 64 * 64);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin in blockIdx order, and
@@ -166,15 +288,25 @@ void Stage::upload(hipStream_t st) {
     d_crows.ensure(crows.size() * sizeof(int64_t));
     QLG_HIP(hipMemcpyAsync(d_crows.p, crows.data(), crows.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
   }
+  // the launch lists: the split ones when a task is split
+  const std::vector<GemmTask> &lt = split() ? ktasks : tasks;
+  const std::vector<GemmSeg> &ls = split() ? ksegs : segs;
   d_tiles.ensure(std::max<size_t>(1, tiles.size()) * sizeof(GemmTile));
-  d_tasks.ensure(std::max<size_t>(1, tasks.size()) * sizeof(GemmTask));
-  d_segs.ensure(std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
+  d_tasks.ensure(std::max<size_t>(1, lt.size()) * sizeof(GemmTask));
+  d_segs.ensure(std::max<size_t>(1, ls.size()) * sizeof(GemmSeg));
   if (!tiles.empty())
     QLG_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(GemmTile), hipMemcpyHostToDevice, st));
-  if (!tasks.empty())
-    QLG_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(GemmTask), hipMemcpyHostToDevice, st));
-  if (!segs.empty())
-    QLG_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(GemmSeg), hipMemcpyHostToDevice, st));
+  if (!lt.empty())
+    QLG_HIP(hipMemcpyAsync(d_tasks.p, lt.data(), lt.size() * sizeof(GemmTask), hipMemcpyHostToDevice, st));
+  if (!ls.empty())
+    QLG_HIP(hipMemcpyAsync(d_segs.p, ls.data(), ls.size() * sizeof(GemmSeg), hipMemcpyHostToDevice, st));
+  if (split()) {
+    d_red.ensure(red.size() * sizeof(RedTask));
+    d_redc.ensure(redc.size() * sizeof(RedChunk));
+    work.ensure((size_t)std::max<int64_t>(work_elems, 1) * ed * sizeof(double));
+    QLG_HIP(hipMemcpyAsync(d_red.p, red.data(), red.size() * sizeof(RedTask), hipMemcpyHostToDevice, st));
+    QLG_HIP(hipMemcpyAsync(d_redc.p, redc.data(), redc.size() * sizeof(RedChunk), hipMemcpyHostToDevice, st));
+  }
 }
 
 // Bounds of every access a task list can make: rows < M, cols < N, k < K of

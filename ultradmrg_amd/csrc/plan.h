@@ -27,8 +27,24 @@ struct Stage {
   double flops = 0;      // useful 2*M*N*K issued by this stage
   double algo_flops = 0; // this stage's share of the reference schedule's FLOPs
   double algo_bytes = 0; // operands read once + results written once (bytes)
+  // Split-K (the matvec's stage 3): with `splitk` and `split_on` set, make_tiles cuts tasks
+  // whose K is long into parts of at most `kchunk` (a multiple of 64), which
+  // run as the launch list ktasks / ksegs and write partial panels to `work`;
+  // `red` / `redc` drive the reduce kernel.  tasks / segs stay the unsplit
+  // lists (signatures, batch plans, the cost model).
+  bool splitk = false;             // the stage may run split-K parts
+  bool split_on = false;           // make_tiles splits (the tuner's choice, or QLG_SPLITK)
+  int ed = 1;                      // doubles per element (complex: 2)
+  int64_t kchunk = 0;              // 0: no task is split
+  std::vector<GemmTask> ktasks;
+  std::vector<GemmSeg> ksegs;
+  std::vector<RedTask> red;
+  std::vector<RedChunk> redc;
+  int64_t work_elems = 0;
+  DevBuf d_red, d_redc, work;
   void upload(hipStream_t st);
   void make_tiles(int bm_, int bn_);
+  bool split() const { return kchunk > 0 && !red.empty(); }
 };
 
 struct Layout {
@@ -161,5 +177,9 @@ std::string tile_signature(const Stage &s, int stage, int variant);
 
 void launch_grouped_gemm_f64(int bm, int bn, bool cplx, const GemmTile *tiles, const GemmTask *tasks,
                              const GemmSeg *segs, const Slots &slots, int ntiles, hipStream_t st);
+// sums the split-K partial panels of a stage into its outputs (FP64 / complex128)
+void launch_splitk_reduce(bool cplx, const RedTask *red, const RedChunk *chunks, int nchunks, const Slots &slots,
+                          hipStream_t st);
+int64_t pick_kchunk(const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
 
 }  // namespace qlg

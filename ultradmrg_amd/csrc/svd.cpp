@@ -33,6 +33,7 @@
 #include "svd.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -171,6 +172,19 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
   hipStream_t stream = c.stream;
   const char *dbg = std::getenv("QLG_SVD_DEBUG");
   const bool debug = dbg && dbg[0] == '1';
+  // QLG_SVD_PROFILE=1: one line per call on stderr with the sector shapes and
+  // the wall time of each phase (the stream is drained at every phase end)
+  const char *prf = std::getenv("QLG_SVD_PROFILE");
+  const bool profile = prf && prf[0] == '1';
+  std::vector<std::pair<const char *, double>> phase;
+  auto t_last = std::chrono::steady_clock::now();
+  auto mark = [&](const char *name) {
+    if (!profile) return;
+    QLG_HIP(hipStreamSynchronize(stream));
+    const auto now = std::chrono::steady_clock::now();
+    phase.push_back({name, std::chrono::duration<double, std::milli>(now - t_last).count()});
+    t_last = now;
+  };
   QN div{0, 0};
   for (int k = 0; k < t.qn_len; ++k) div[k] = ldiv[k];
 
@@ -318,6 +332,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     upload(c, dmats2, pm);
     svd_pad_diag(stream, dmats2.as<SvdMat>(), (int64_t)pm.size(), G.as<double>(), ed);
   }
+  mark("gram");
 
   // ---- eigen-decompositions (rows of E^H = eigenvectors^H, ascending) ------------------
   std::vector<char> forced(sec.size(), 0);   // sectors left to the Jacobi refinement (E = I)
@@ -432,6 +447,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       }
   }
 
+  mark("eigh");
   // ---- back-projection R = E^H X (valid eigenvectors: the last n rows) ------------------
   {
     GemmBatch gb(c, cplx);
@@ -496,6 +512,8 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
   double err = 0.0;
   int64_t D = 0;
   truncate(scaled_values(), keep, err, D);
+  mark("project");
+  int jac_sweeps = 0;
   {
     // Sectors whose kept values reach below 1e-6 of their largest: the rows of
     // such small values carry components of their neighbours' directions
@@ -559,6 +577,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
         for (int32_t f : hflag) any |= f != 0;
         if (debug) std::fprintf(stderr, "[qlg svd] Jacobi pass %d sweep %d: %zu jobs, rotated %d\n", pass, sweep,
                                 jobs.size(), (int)any);
+        ++jac_sweeps;
         if (!any) break;
       }
       svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
@@ -575,6 +594,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
       for (int64_t i = 0; i < s.n; ++i) std::fprintf(stderr, " %.3e", hsv[s.sv0 + i]);
       std::fprintf(stderr, "\n");
     }
+  mark("refine");
   // ---- kept rows: P = R_j / s_j in descending order, U side from E^H ----------------------
   // per kept sector: perm (descending order of its rows), factors 1 / s
   std::vector<int> kept;
@@ -635,6 +655,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
                   G.as<double>(), EK.as<double>(), ed);
 
   // ---- Newton-Schulz on the kept rows of P --------------------------------------------------
+  int ns_rounds = 0;
   {
     std::vector<int64_t> abase(kept.size());
     int64_t atot = 0;
@@ -648,6 +669,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     std::vector<int> active(kept.size());
     std::iota(active.begin(), active.end(), 0);
     for (int round = 0; round < 8 && !active.empty(); ++round) {
+      ns_rounds = round + 1;
       GemmBatch g1(c, cplx);
       std::vector<SvdMat> am;
       for (int i : active) {
@@ -721,6 +743,7 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     }
   }
 
+  mark("orthonormalise");
   // ---- outputs -----------------------------------------------------------------------------------
   auto *res = new SvdResult();
   std::unique_ptr<SvdResult> guard(res);
@@ -832,6 +855,33 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     for (const SvdRows &j : dj) kmax = std::max(kmax, j.k * j.k);
     svd_diag(stream, dd.as<SvdRows>(), (int64_t)dj.size(), kmax, dsv.as<double>(), res->data[1].as<double>());
     QLG_HIP(hipStreamSynchronize(stream));
+  }
+  mark("assemble");
+  if (profile) {
+    int64_t nmax = 0, mmax = 0, nsum = 0;
+    for (const Sector &s : sec) {
+      nmax = std::max(nmax, s.n);
+      mmax = std::max(mmax, s.m);
+      nsum += s.n;
+    }
+    double tot = 0.0;
+    std::string line;
+    char buf[96];
+    for (auto &ph : phase) {
+      std::snprintf(buf, sizeof buf, " %s %.2f", ph.first, ph.second);
+      line += buf;
+      tot += ph.second;
+    }
+    std::string bl;
+    for (const Bucket &B : bk) {
+      std::snprintf(buf, sizeof buf, " %lldx%zu", (long long)B.b, B.secs.size());
+      bl += buf;
+    }
+    std::fprintf(stderr,
+                 "[qlg svd prof] sectors %zu n_max %lld m_max %lld n_sum %lld D %lld buckets%s | ms%s total %.2f "
+                 "| jacobi sweeps %d ns rounds %d\n",
+                 sec.size(), (long long)nmax, (long long)mmax, (long long)nsum, (long long)D, bl.c_str(), line.c_str(),
+                 tot, jac_sweeps, ns_rounds);
   }
   return guard.release();
 }
