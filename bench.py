@@ -48,13 +48,6 @@ import os
 import sys
 import time
 
-# The block SVD spreads its eigen-decompositions over four streams (each a
-# chain of small, latency-bound launches).  With HIP's default of four
-# hardware queues per process those streams share queues with the solver's
-# own streams and partly serialise; eight keep them apart (set before the
-# first HIP call; a value already in the environment wins).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
