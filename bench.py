@@ -74,6 +74,45 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "GBps": 2 * nbytes / el / 1e9}), flush=True)
 
 
+def vector_kernels(D=2000, reps=50):
+    """Krylov vector kernels on vectors of the D=2000 sizes, for a kernel
+    trace (rocprofv3 --kernel-trace --stats): the library's dot, nrm2, axpy,
+    scal and linear combination (k = 8) on the converged Hubbard state's
+    two-site vector size (3,288,917 elements), ``reps`` calls each, then a
+    30-iteration Lanczos solve on the synthetic D=2000 workload (its fused
+    residual, check and normalisation kernels).  Prints the host-clock time
+    per call (each call returns after completion) and the element counts the
+    trace's durations are divided by."""
+    import torch
+    from ultradmrg_amd import tensor as T
+    from ultradmrg_amd.solver import EffectiveHamiltonian, LanczosParams
+    from ultradmrg_amd.workloads import synthetic_eff_ham
+    n = 3288917
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    y = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    bases = [torch.randn(n, dtype=torch.float64, device="cuda", generator=g) for _ in range(8)]
+    ops = {"dot": (lambda: T.dot(x, y), 2), "nrm2": (lambda: T.nrm2(x), 1),
+           "axpy": (lambda: T.axpy(1e-3, x, y), 3), "scal": (lambda: T.scal(1.0, x), 2),
+           "linear_combine_k8": (lambda: T.linear_combine([0.1] * 8, bases, 0.0, y), 9)}
+    out = {"n": n, "element_bytes": 8, "per_call": {}}
+    for name, (fn, touches) in ops.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t0) / reps * 1e6
+        out["per_call"][name] = {"us": us, "bytes": touches * n * 8, "GBps_per_call": touches * n * 8 / us / 1e3}
+    d_eff, x0 = synthetic_eff_ham("hubbard", D, seed=1003)
+    H = EffectiveHamiltonian(d_eff, x0)
+    res = H.lanczos(x0, LanczosParams(1e-30, 30))
+    torch.cuda.synchronize()
+    out["lanczos"] = {"n": int(x0.data.numel()), "iterations": res.iters}
+    print(json.dumps(out), flush=True)
+
+
 def recorded_traffic(D, precision, stage):
     """HBM bytes per launch of `stage` from the recorded counter runs (or None)."""
     if D != 2000 or precision != "fp64" or not os.path.exists(TRAFFIC_FILE):
@@ -457,6 +496,8 @@ def main():
     ap.add_argument("--D", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--vector-kernels", action="store_true",
+                    help="only the Krylov vector kernels at the D=2000 sizes (for a kernel trace)")
     ap.add_argument("--split", default="bf16x6", help="reduced-precision mode compared with FP64")
     ap.add_argument("--split-alt", default="bf16x3",
                     help="second reduced-precision mode measured beside it in the matvec and sweep ('none': skip)")
@@ -474,6 +515,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.vector_kernels:
+        vector_kernels()
         return
     if a.sweep_profile:
         # one sweep of each kind at D=2000 after the ramp, for a profiler run
