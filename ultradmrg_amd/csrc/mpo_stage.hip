@@ -251,6 +251,36 @@ __global__ __launch_bounds__(256) void trans_copy_kernel(const TransDesc *__rest
   }
 }
 
+// the same copies with a source buffer per descriptor (many operators, one launch)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void trans_copy_multi_kernel(const TransDesc *__restrict__ d,
+                                                               const double *const *__restrict__ srcs,
+                                                               double *__restrict__ dst, int n) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const TransDesc c = d[i];
+  const double *src = srcs[i];
+  const int64_t tot = (int64_t)c.nz * c.nx;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) {
+    const int64_t x = e / c.nz, z = e % c.nz;
+    if constexpr (CPLX)
+      reinterpret_cast<double2 *>(dst)[c.dst + x * c.dsx + z] =
+          reinterpret_cast<const double2 *>(src)[c.src + z * c.sld + x];
+    else
+      dst[c.dst + x * c.dsx + z] = src[c.src + z * c.sld + x];
+  }
+}
+
+void launch_trans_copy_multi(bool cplx, const TransDesc *d, const double *const *srcs, int n, double *dst,
+                             hipStream_t st) {
+  if (n <= 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(trans_copy_multi_kernel<true>, dim3(n), dim3(256), 0, st, d, srcs, dst, n);
+  else
+    hipLaunchKernelGGL(trans_copy_multi_kernel<false>, dim3(n), dim3(256), 0, st, d, srcs, dst, n);
+  QLG_HIP(hipGetLastError());
+}
+
 void launch_trans_copy(bool cplx, const TransDesc *d, int n, const double *src, double *dst, hipStream_t st) {
   if (n <= 0) return;
   if (cplx)

@@ -460,8 +460,16 @@ std::vector<int64_t> sector_offsets(const Axis &a) {
   return o;
 }
 
-// dense host copy of a small tensor (MPO), index order kept
-std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed) {
+// elements spanned by a tensor's blocks from its data pointer
+int64_t raw_elems(const Ten &t) {
+  int64_t raw = 0;
+  for (size_t b = 0; b < t.blk.size(); ++b) raw = std::max(raw, t.off[b] + t.blk_elems(b));
+  return raw;
+}
+
+// dense host array of a small tensor from a host copy of its block data
+// (raw_elems(t) elements), index order kept
+std::vector<double> densify(const Ten &t, const double *h, int ed) {
   std::vector<int64_t> dims(t.rank);
   std::vector<std::vector<int64_t>> so(t.rank);
   int64_t tot = 1;
@@ -469,13 +477,6 @@ std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed) {
     dims[i] = t.ax[i].dim();
     so[i] = sector_offsets(t.ax[i]);
     tot *= dims[i];
-  }
-  int64_t raw = 0;
-  for (size_t b = 0; b < t.blk.size(); ++b) raw = std::max(raw, t.off[b] + t.blk_elems(b));
-  std::vector<double> h(raw * ed);
-  if (raw > 0) {
-    QLG_HIP(hipMemcpyAsync(h.data(), t.data, raw * ed * sizeof(double), hipMemcpyDeviceToHost, st));
-    QLG_HIP(hipStreamSynchronize(st));
   }
   std::vector<double> d(tot * ed, 0.0);
   for (size_t b = 0; b < t.blk.size(); ++b) {
@@ -491,6 +492,17 @@ std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed) {
     }
   }
   return d;
+}
+
+// dense host copy of a small tensor (MPO), index order kept
+std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed) {
+  const int64_t raw = raw_elems(t);
+  std::vector<double> h(raw * ed);
+  if (raw > 0) {
+    QLG_HIP(hipMemcpyAsync(h.data(), t.data, raw * ed * sizeof(double), hipMemcpyDeviceToHost, st));
+    QLG_HIP(hipStreamSynchronize(st));
+  }
+  return densify(t, h.data(), ed);
 }
 
 

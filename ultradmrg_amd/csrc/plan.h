@@ -155,6 +155,8 @@ Plan *build_env_plan(Ctx *ctx, const qlg_bsten *env, const qlg_bsten *mps, const
 void check_pair(const Axis &a, const Axis &b, const char *what);
 std::vector<int64_t> sector_offsets(const Axis &a);
 std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed);
+int64_t raw_elems(const Ten &t);
+std::vector<double> densify(const Ten &t, const double *host_raw, int ed);
 void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
 void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const GemmTile *tiles,
                       const ChanTask *tasks, const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip,

@@ -24,7 +24,8 @@
 
 namespace qlg {
 
-void launch_trans_copy(bool cplx, const TransDesc *d, int n, const double *src, double *dst, hipStream_t st);
+void launch_trans_copy_multi(bool cplx, const TransDesc *d, const double *const *srcs, int n, double *dst,
+                             hipStream_t st);
 
 namespace {
 
@@ -225,12 +226,17 @@ Plan *build_super_block_plan(Ctx *ctx, int64_t ngroups, const int64_t *nleft, co
   QLG_HIP(hipMemsetAsync(bufR->p, 0, std::max<int64_t>(R.size, 1) * ed * sizeof(double), st));
   L.d.data = bufL->p;
   R.d.data = bufR->p;
-  // transposed copies: op[z][x] -> stacked[x][pos][z]
-  auto stack = [&](const std::vector<Ten> &ops, const BondIndex &bi, const Built &B, const Axis &ax_row,
+  // transposed copies: op[z][x] -> stacked[x][pos][z], every block of every
+  // operator in one launch (a source pointer per block); the descriptor
+  // buffers live until the stream is drained below
+  DevBuf dd[2], dsrc[2];
+  std::vector<TransDesc> hv[2];
+  std::vector<const double *> hs[2];
+  auto stack = [&](int side, const std::vector<Ten> &ops, const BondIndex &bi, const Built &B, const Axis &ax_row,
                    const Axis &ax_col, double *dst) {
-    // every op shares one source buffer layout; copies run one launch per operator
+    std::vector<TransDesc> &v = hv[side];
+    std::vector<const double *> &srcs = hs[side];
     for (size_t o = 0; o < ops.size(); ++o) {
-      std::vector<TransDesc> v;
       const Ten &t = ops[o];
       const int64_t degw = bi.sect_deg[bi.el_sect[o]];
       for (size_t b = 0; b < t.blk.size(); ++b) {
@@ -245,17 +251,45 @@ Plan *build_super_block_plan(Ctx *ctx, int64_t ngroups, const int64_t *nleft, co
         d.nz = (int32_t)nz;
         d.nx = (int32_t)nx;
         v.push_back(d);
+        srcs.push_back(static_cast<const double *>(t.data));
       }
-      if (v.empty()) continue;
-      DevBuf dd;
-      dd.ensure(v.size() * sizeof(TransDesc));
-      QLG_HIP(hipMemcpyAsync(dd.p, v.data(), v.size() * sizeof(TransDesc), hipMemcpyHostToDevice, st));
-      launch_trans_copy(ed == 2, dd.as<TransDesc>(), (int)v.size(), static_cast<const double *>(t.data), dst, st);
-      QLG_HIP(hipStreamSynchronize(st));   // dd is released at scope end
     }
+    if (v.empty()) return;
+    QLG_CHECK(v.size() < (size_t(1) << 31), "super block: too many operator blocks");
+    dd[side].ensure(v.size() * sizeof(TransDesc));
+    dsrc[side].ensure(srcs.size() * sizeof(const double *));
+    QLG_HIP(hipMemcpyAsync(dd[side].p, v.data(), v.size() * sizeof(TransDesc), hipMemcpyHostToDevice, st));
+    QLG_HIP(hipMemcpyAsync(dsrc[side].p, srcs.data(), srcs.size() * sizeof(const double *), hipMemcpyHostToDevice,
+                           st));
+    launch_trans_copy_multi(ed == 2, dd[side].as<TransDesc>(), dsrc[side].as<const double *>(), (int)v.size(), dst,
+                            st);
   };
-  stack(Ls, W, L, ax_a, ax_ap, bufL->as<double>());
-  stack(Rs, Z, R, ax_b, ax_bp, bufR->as<double>());
+  stack(0, Ls, W, L, ax_a, ax_ap, bufL->as<double>());
+  stack(1, Rs, Z, R, ax_b, ax_bp, bufR->as<double>());
+  // ---- host copies of the site operators: one device-to-device gather of their
+  // block data into a staging buffer, one copy to the host, one wait ----------------------
+  std::map<const Ten *, int64_t> site_base;   // staging offset (elements) of each term's site operator
+  int64_t stage_n = 0;
+  for (auto *grp : {&gl, &gr})
+    for (auto &g : *grp)
+      for (auto &t : g) {
+        site_base[&t.site] = stage_n;
+        stage_n += raw_elems(t.site);
+      }
+  std::vector<double> stage_h((size_t)std::max<int64_t>(stage_n, 1) * ed);
+  {
+    DevBuf stage_d;
+    stage_d.ensure((size_t)std::max<int64_t>(stage_n, 1) * ed * sizeof(double));
+    for (auto &kv : site_base) {
+      const int64_t n = raw_elems(*kv.first);
+      if (n > 0)
+        QLG_HIP(hipMemcpyAsync(stage_d.as<double>() + kv.second * ed, kv.first->data, n * ed * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
+    }
+    if (stage_n > 0)
+      QLG_HIP(hipMemcpyAsync(stage_h.data(), stage_d.p, stage_n * ed * sizeof(double), hipMemcpyDeviceToHost, st));
+    QLG_HIP(hipStreamSynchronize(st));   // also completes the stacking launches
+  }
   // ---- W_l[w, s, s', y] and W_r[y, s, s', z] (small: assembled on the host) ------------------
   auto site_tensor = [&](Built &B, const BondIndex &bin, const BondIndex &bout,
                          const std::vector<std::vector<T2>> &groups, bool left_side, std::vector<double> &host) {
@@ -277,7 +311,7 @@ Plan *build_super_block_plan(Ctx *ctx, int64_t ngroups, const int64_t *nleft, co
       for (const T2 &t : groups[j]) {
         const int64_t ein = left_side ? t.op : j, eout = left_side ? j : t.op;
         if ((left_side ? lempty[t.op] : rempty[t.op]) || t.site.blk.empty()) continue;
-        Entry e{ein, eout, dense_copy(t.site, st, ed)};
+        Entry e{ein, eout, densify(t.site, stage_h.data() + site_base.at(&t.site) * ed, ed)};
         for (const Coords &c : t.site.blk)
           blocks.insert(Coords{bin.el_sect[ein], c[0], c[1], bout.el_sect[eout]});
         entries.push_back(std::move(e));
