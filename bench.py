@@ -113,6 +113,89 @@ def vector_kernels(D=2000, reps=50):
     print(json.dumps(out), flush=True)
 
 
+def svd_profile(D=2000):
+    """Phase times of the block SVD over one FP64 D=2000 sweep of the 4x16
+    Hubbard cylinder (after a short ramp: two sweeps at D = 10, 100, 1000,
+    three at D=2000).  The library's QLG_SVD_PROFILE=1 mode drains the stream
+    at every phase end and each eigensolver lane at its end, and prints one
+    line per call; they are read back from the process's stderr and summed:
+    per phase, the eigensolver lanes' finishing times, and the sector shapes."""
+    import re
+    import tempfile
+    import torch
+    from ultradmrg_amd import mpo as M
+    from ultradmrg_amd.solver import LanczosParams
+    from ultradmrg_amd.sweep import SweepParams, TwoSiteVMPS, direct_state_mps
+    m = M.hubbard_cylinder(16, 4, t=1.0, U=8.0)
+    per_hole, labs, sz = m.n // 8, [], 0
+    for i in range(m.n):
+        if i % per_hole == per_hole - 1:
+            labs.append(3)
+        else:
+            labs.append(sz % 2 + 1)
+            sz += 1
+    lp = LanczosParams(1e-9, 100)
+    drv = TwoSiteVMPS(direct_state_mps(m, labs), M.build_mpo(m), SweepParams(1, 10, 10, 1e-8, lp, verbose=False))
+    for Dr, ns in ((10, 2), (100, 2), (1000, 2), (D, 3)):
+        drv.p = SweepParams(1, Dr, Dr, 1e-8, lp, verbose=False)
+        for k in range(ns):
+            e = drv.sweep()
+            print(f"[bench] svd profile ramp D={Dr} sweep {k + 1}: E = {e:.10f}", file=sys.stderr, flush=True)
+    st = drv.stats
+    for k in ("lanczos_s", "svd_s", "env_s", "contract_s", "plan_s"):
+        setattr(st, k, 0)
+    torch.cuda.synchronize()
+    sys.stderr.flush()
+    with tempfile.TemporaryFile(mode="w+") as tmp:
+        saved = os.dup(2)
+        os.dup2(tmp.fileno(), 2)
+        os.environ["QLG_SVD_PROFILE"] = "1"
+        os.environ["QLG_VERBOSE"] = "2"   # the planner's host phase times
+        try:
+            e = drv.sweep()
+            torch.cuda.synchronize()
+        finally:
+            os.environ.pop("QLG_SVD_PROFILE", None)
+            os.environ.pop("QLG_VERBOSE", None)
+            sys.stderr.flush()
+            os.dup2(saved, 2)
+            os.close(saved)
+        tmp.seek(0)
+        lines = tmp.read().splitlines()
+    plans = [l for l in lines if l.startswith("[qlg plan ms]")]
+    plan_ms = {}
+    for l in plans:
+        for k, v in re.findall(r" ([a-z0-9+]+) ([\d.]+)", l.split("]", 1)[1]):
+            plan_ms[k] = plan_ms.get(k, 0.0) + float(v)
+    calls = [l for l in lines if l.startswith("[qlg svd prof]")]
+    lanes = [l for l in lines if l.startswith("[qlg eigh prof]")]
+    phases = {}
+    nmax = []
+    for l in calls:
+        for k, v in re.findall(r" (gram|eigh|project|refine|orthonormalise|assemble) ([\d.]+)", l):
+            phases[k] = phases.get(k, 0.0) + float(v) / 1e3
+        mm = re.search(r"n_max (\d+)", l)
+        if mm:
+            nmax.append(int(mm.group(1)))
+    lane_max = [max(float(x) for x in re.findall(r"\] ([\d.]+) ms", l)) for l in lanes]
+    lane_big = []
+    for l in lanes:   # finishing time of the lane holding the largest matrices
+        groups = re.findall(r"lane \d+ \[([^\]]*)\] ([\d.]+) ms", l)
+        best = max(groups, key=lambda g: max(int(x.split("x")[0]) for x in g[0].split()))
+        lane_big.append(float(best[1]))
+    sweep_stats = {k: getattr(st, k) for k in ("lanczos_s", "svd_s", "env_s", "contract_s", "plan_s")}
+    out = {"workload": f"Hubbard 16x4 cylinder, one FP64 sweep at D={D}", "energy": e, "svd_calls": len(calls),
+           "sweep_s": st.sweep_seconds[-1], "sweep_phase_s": sweep_stats, "plans": len(plans),
+           "plan_phase_ms_total": plan_ms,
+           "svd_s": st.svd_s, "phase_s": phases, "largest_sector_order_median": sorted(nmax)[len(nmax) // 2],
+           "eigh_lanes_critical_ms_median": sorted(lane_max)[len(lane_max) // 2] if lane_max else None,
+           "eigh_largest_bucket_lane_ms_median": sorted(lane_big)[len(lane_big) // 2] if lane_big else None,
+           "largest_call": max(calls, key=lambda l: int(re.search(r"n_max (\d+)", l).group(1))) if calls else None,
+           "slowest_lanes": max(lanes, key=lambda l: max(float(x) for x in re.findall(r"\] ([\d.]+) ms", l)))
+           if lanes else None}
+    print(json.dumps(out), flush=True)
+
+
 def recorded_traffic(D, precision, stage):
     """HBM bytes per launch of `stage` from the recorded counter runs (or None)."""
     if D != 2000 or precision != "fp64" or not os.path.exists(TRAFFIC_FILE):
@@ -496,6 +579,8 @@ def main():
     ap.add_argument("--D", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--svd-profile", action="store_true",
+                    help="only the block SVD's phase times over one D=2000 sweep (QLG_SVD_PROFILE)")
     ap.add_argument("--vector-kernels", action="store_true",
                     help="only the Krylov vector kernels at the D=2000 sizes (for a kernel trace)")
     ap.add_argument("--split", default="bf16x6", help="reduced-precision mode compared with FP64")
@@ -518,6 +603,9 @@ def main():
         return
     if a.vector_kernels:
         vector_kernels()
+        return
+    if a.svd_profile:
+        svd_profile()
         return
     if a.sweep_profile:
         # one sweep of each kind at D=2000 after the ramp, for a profiler run
