@@ -101,7 +101,9 @@ void Stage::make_tiles(int bm_, int bn_) {
       const GemmTask &t = tasks[ti];
       int64_t ktot = 0;
       for (int q = 0; q < t.seg_count; ++q) ktot += segs[t.seg_begin + q].K;
-      if (ktot * 2 <= kchunk * 3 || t.M <= 0 || t.N <= 0) {
+      // tasks too short to cut, and panels too large for the reduce
+      // kernel's 32-bit element index, run unsplit
+      if (ktot * 2 <= kchunk * 3 || t.M <= 0 || t.N <= 0 || (int64_t)t.M * t.N >= (int64_t(1) << 31)) {
         GemmTask c = t;
         c.seg_begin = (int32_t)ksegs.size();
         for (int q = 0; q < t.seg_count; ++q) ksegs.push_back(segs[t.seg_begin + q]);
@@ -176,8 +178,12 @@ void Stage::make_tiles(int bm_, int bn_) {
       for (int64_t e = 0; e < panel; e += kRedChunk) redc.push_back(RedChunk{(int32_t)red.size(), (int32_t)e});
       red.push_back(r);
     }
-    QLG_CHECK(work_elems < (int64_t(1) << 31) / 2, "split-K workspace too large");
-    if (red.empty()) {
+    // a workspace beyond 2^29 elements (4 GiB in FP64, 8 GiB complex) is not
+    // worth its traffic: such a stage runs unsplit
+    if (red.empty() || work_elems >= (int64_t(1) << 29)) {
+      red.clear();
+      redc.clear();
+      work_elems = 0;
       kchunk = 0;
       ktasks.clear();
       ksegs.clear();
