@@ -165,7 +165,7 @@ def svd_profile(D=2000):
     plans = [l for l in lines if l.startswith("[qlg plan ms]")]
     plan_ms = {}
     for l in plans:
-        for k, v in re.findall(r" ([a-z0-9+]+) ([\d.]+)", l.split("]", 1)[1]):
+        for k, v in re.findall(r" ([a-z0-9+-]+) ([\d.]+)", l.split("]", 1)[1]):
             plan_ms[k] = plan_ms.get(k, 0.0) + float(v)
     calls = [l for l in lines if l.startswith("[qlg svd prof]")]
     lanes = [l for l in lines if l.startswith("[qlg eigh prof]")]

@@ -447,6 +447,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     std::vector<InRow> in;
     std::vector<Key7> out;
   };
+  pt.mark("s2-mpo");
   std::map<std::pair<int, int>, int> cid;
   std::vector<Chan> ch;
   for (auto &kv : lst) {
@@ -492,6 +493,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   // Combined two-site operator, independent of the channel:
   // Wc[(w, s1, t0), (s2, t1, z)] = sum_y Wl[w, s1, t0, y] Wr[y, s2, t1, z],
   // formed once from the nonzero entries of the (sparse) MPO tensors.
+  pt.mark("s2-rows");
   std::vector<double> Wc;
   int64_t wc_cols = 0;
   if (nphys == 2) {
@@ -518,6 +520,7 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         }
       }
   }
+  pt.mark("s2-wc");
   std::vector<double> Ch;
   int64_t coff = 0;
   for (Chan &c : ch) {
