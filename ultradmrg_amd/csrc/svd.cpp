@@ -37,6 +37,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <map>
 #include <numeric>
 
@@ -526,59 +527,217 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     constexpr double kTau = 1e-6;
     std::vector<char> refined(sec.size(), 0);
     for (int pass = 0; pass < 8; ++pass) {
-      // jobs: the sector's rows above its rounding floor (64 eps of its largest
-      // value; rows below it are reported as zero and completed anyway),
-      // largest first
-      std::vector<SvdJacB> jobs;
-      std::vector<int32_t> idx, pairs;
-      int maxnbp = 0;
+      // sectors to refine in this pass
+      std::vector<int> active;
       for (size_t si = 0; si < sec.size(); ++si) {
         const Sector &s = sec[si];
         if (refined[si] || s.n < 2) continue;
         if (keep[si] == 0 && !forced[si]) continue;
-        std::vector<int32_t> ord(s.n);
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(),
-                         [&](int32_t a, int32_t b) { return hsv[s.sv0 + a] > hsv[s.sv0 + b]; });
-        const double vmax = hsv[s.sv0 + ord[0]];
-        if (!forced[si] && hsv[s.sv0 + ord[keep[si] - 1]] >= kTau * vmax) continue;
+        double vmax = 0.0, vkeep = 1e300;
+        std::vector<double> v(hsv.begin() + s.sv0, hsv.begin() + s.sv0 + s.n);
+        std::vector<double> sorted = v;
+        std::sort(sorted.begin(), sorted.end(), std::greater<double>());
+        vmax = sorted[0];
+        if (keep[si] > 0) vkeep = sorted[keep[si] - 1];
+        if (!forced[si] && vkeep >= kTau * vmax) continue;
         refined[si] = 1;
-        const double floor = 64.0 * 2.220446049250313e-16 * vmax;
-        int64_t k = 0;
-        while (k < s.n && hsv[s.sv0 + ord[k]] > floor) ++k;
-        if (k < 2) continue;
-        const Bucket &B = bk[s.bucket];
-        const int nblk = (int)((k + 15) / 16), nbp = nblk + (nblk & 1);
-        const int job = (int)jobs.size();
-        jobs.push_back(SvdJacB{B.rbase + (int64_t)s.slot * B.b * B.mp, B.mp, s.m,
-                               B.gbase + (int64_t)s.slot * B.b * B.b + (B.b - s.n) * B.b, B.b, k,
-                               (int64_t)idx.size(), nblk, 0});
-        idx.insert(idx.end(), ord.begin(), ord.begin() + k);
-        for (int pi = 0; pi < nbp / 2; ++pi) {
-          pairs.push_back(job);
-          pairs.push_back(pi);
-        }
-        maxnbp = std::max(maxnbp, nbp);
+        active.push_back((int)si);
       }
-      if (jobs.empty()) break;
+      if (active.empty()) break;
+      // Multi-level Gram (real data).  The Gram eigenvectors resolve values
+      // down to ~1e-8 of a sector's largest; the rows below that carry their
+      // neighbours' directions.  In each refined sector, the rows below
+      // 1e-4 (then 1e-8) of its largest value and above its rounding floor
+      // are scaled to unit largest norm, and rotated by the eigenvectors of
+      // their own Gram matrix (the same rotation on their rows of E^H keeps
+      // X = E R exact).  Each level resolves another ~8 decades, so the
+      // Jacobi sweeps below start from nearly orthogonal rows.
+      if (!cplx) {
+        constexpr double kLevelTau = 1e-4;
+        for (int level = 0; level < 2; ++level) {
+          struct Sel {
+            int si;
+            int64_t k, kb, perm, rs, es, gs, ws;
+          };
+          std::vector<Sel> sels;
+          std::vector<int32_t> perm;
+          std::vector<double> fdown, fup;
+          int64_t rs_tot = 0, es_tot = 0;
+          const double thr_rel = level == 0 ? kLevelTau : kLevelTau * kLevelTau;
+          for (int si : active) {
+            const Sector &s = sec[si];
+            const Bucket &B = bk[s.bucket];
+            double vmax = 0.0;
+            for (int64_t r = 0; r < s.n; ++r) vmax = std::max(vmax, hsv[s.sv0 + r]);
+            const double floor = 64.0 * 2.220446049250313e-16 * vmax, thr = thr_rel * vmax;
+            std::vector<int32_t> rows;
+            double smax = 0.0;
+            for (int64_t r = 0; r < s.n; ++r) {
+              const double v = hsv[s.sv0 + r];
+              if (v > floor && v < thr) {
+                rows.push_back((int32_t)r);
+                smax = std::max(smax, v);
+              }
+            }
+            if (rows.size() < 2 || !(smax > 0.0)) continue;
+            Sel q{};
+            q.si = si;
+            q.k = (int64_t)rows.size();
+            q.kb = up16(q.k);
+            q.perm = (int64_t)perm.size();
+            q.rs = rs_tot;
+            rs_tot += q.k * s.m;
+            q.es = es_tot;
+            es_tot += q.k * B.b;
+            perm.insert(perm.end(), rows.begin(), rows.end());
+            fdown.insert(fdown.end(), rows.size(), 1.0 / smax);
+            fup.insert(fup.end(), rows.size(), smax);
+            sels.push_back(q);
+          }
+          if (sels.empty()) break;
+          // Gram matrices grouped by padded order (eigh batches are contiguous)
+          std::map<int64_t, std::vector<size_t>> byk;
+          for (size_t i = 0; i < sels.size(); ++i) byk[sels[i].kb].push_back(i);
+          int64_t gs_tot = 0, ws_tot = 0;
+          for (auto &kv : byk)
+            for (size_t i : kv.second) {
+              sels[i].gs = gs_tot;
+              gs_tot += kv.first * kv.first;
+              sels[i].ws = ws_tot;
+              ws_tot += kv.first;
+            }
+          DevBuf Rs, Rs2, Es, Es2, Gs, Ws, dperm, dfd, dfu, dgr, dge, dpm;
+          Rs.ensure((size_t)rs_tot * 8);
+          Rs2.ensure((size_t)rs_tot * 8);
+          Es.ensure((size_t)es_tot * 8);
+          Es2.ensure((size_t)es_tot * 8);
+          Gs.ensure((size_t)gs_tot * 8);
+          Ws.ensure((size_t)ws_tot * 8);
+          upload(c, dperm, perm);
+          upload(c, dfd, fdown);
+          upload(c, dfu, fup);
+          std::vector<SvdRows> gr, ge, sr, se;
+          std::vector<SvdMat> pm;
+          int64_t rmax = 0, emax = 0;
+          for (const Sel &q : sels) {
+            const Sector &s = sec[q.si];
+            const Bucket &B = bk[s.bucket];
+            const int64_t r0 = B.rbase + (int64_t)s.slot * B.b * B.mp;
+            const int64_t e0 = B.gbase + (int64_t)s.slot * B.b * B.b + (B.b - s.n) * B.b;
+            gr.push_back(SvdRows{r0, B.mp, q.rs, s.m, q.k, s.m, q.perm});
+            ge.push_back(SvdRows{e0, B.b, q.es, B.b, q.k, B.b, q.perm});
+            sr.push_back(SvdRows{q.rs, s.m, r0, B.mp, q.k, s.m, q.perm});
+            se.push_back(SvdRows{q.es, B.b, e0, B.b, q.k, B.b, q.perm});
+            pm.push_back(SvdMat{q.gs, q.kb, q.k, q.kb, 0});
+            rmax = std::max(rmax, q.k * s.m);
+            emax = std::max(emax, q.k * B.b);
+          }
+          upload(c, dgr, gr);
+          upload(c, dge, ge);
+          svd_gather_rows(stream, dgr.as<SvdRows>(), (int64_t)gr.size(), rmax, dperm.as<int32_t>(), dfd.as<double>(),
+                          R.as<double>(), Rs.as<double>(), 1);
+          svd_gather_rows(stream, dge.as<SvdRows>(), (int64_t)ge.size(), emax, dperm.as<int32_t>(), nullptr,
+                          G.as<double>(), Es.as<double>(), 1);
+          QLG_HIP(hipMemsetAsync(Gs.p, 0, (size_t)gs_tot * 8, stream));
+          {
+            GemmBatch gb(c, false);
+            for (const Sel &q : sels) {
+              const Sector &s = sec[q.si];
+              gb.add(Rs.as<double>() + q.rs, s.m, false, false, Rs.as<double>() + q.rs, s.m, true, false,
+                     Gs.as<double>() + q.gs, q.kb, q.k, q.k, s.m);
+            }
+            gb.run();
+          }
+          upload(c, dpm, pm);
+          svd_pad_diag(stream, dpm.as<SvdMat>(), (int64_t)pm.size(), Gs.as<double>(), 1);
+          std::vector<qlg_eigh_group> groups;
+          for (auto &kv : byk) {
+            const Sel &q0 = sels[kv.second.front()];
+            groups.push_back(qlg_eigh_group{kv.first, (int64_t)kv.second.size(), Gs.as<double>() + q0.gs,
+                                            Ws.as<double>() + q0.ws});
+          }
+          eigh_batched(c, QLG_F64, (int64_t)groups.size(), groups.data());
+          {
+            // rows of the eigenvector buffer kb - k .. kb - 1: the valid (ascending) eigenvectors
+            GemmBatch gb(c, false);
+            for (const Sel &q : sels) {
+              const Sector &s = sec[q.si];
+              const Bucket &B = bk[s.bucket];
+              const double *V = Gs.as<double>() + q.gs + (q.kb - q.k) * q.kb;
+              gb.add(V, q.kb, false, false, Rs.as<double>() + q.rs, s.m, false, false, Rs2.as<double>() + q.rs, s.m,
+                     q.k, s.m, q.k);
+              gb.add(V, q.kb, false, false, Es.as<double>() + q.es, B.b, false, false, Es2.as<double>() + q.es, B.b,
+                     q.k, B.b, q.k);
+            }
+            gb.run();
+          }
+          upload(c, dgr, sr);
+          upload(c, dge, se);
+          svd_scatter_rows(stream, dgr.as<SvdRows>(), (int64_t)sr.size(), rmax, dperm.as<int32_t>(), dfu.as<double>(),
+                           Rs2.as<double>(), R.as<double>(), 1);
+          svd_scatter_rows(stream, dge.as<SvdRows>(), (int64_t)se.size(), emax, dperm.as<int32_t>(), nullptr,
+                           Es2.as<double>(), G.as<double>(), 1);
+          svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
+          download<double>(c, hsv, sv.p, (size_t)nsv);   // waits for the rotations before the buffers go
+        }
+      }
+      // Sweeps.  Before each one a sector's rows are re-sorted by their
+      // current norms (de Rijk's pivoting: rows of like size meet in the same
+      // blocks, which cuts the sweep count on graded spectra), rows at the
+      // sector's rounding floor (64 eps of its largest) are left out, and a
+      // sector whose last sweep rotated nothing is finished.
       DevBuf dj, dpairs, didx, dflag;
-      upload(c, dj, jobs);
-      upload(c, dpairs, pairs);
-      upload(c, didx, idx);
-      dflag.ensure(jobs.size() * sizeof(int32_t));
       std::vector<int32_t> hflag;
-      for (int sweep = 0; sweep < 30; ++sweep) {
+      for (int sweep = 0; sweep < 30 && !active.empty(); ++sweep) {
+        std::vector<SvdJacB> jobs;
+        std::vector<int32_t> idx, pairs;
+        std::vector<int> job_sec;
+        int maxnbp = 0;
+        for (int si : active) {
+          const Sector &s = sec[si];
+          std::vector<int32_t> ord(s.n);
+          std::iota(ord.begin(), ord.end(), 0);
+          std::stable_sort(ord.begin(), ord.end(),
+                           [&](int32_t a, int32_t b) { return hsv[s.sv0 + a] > hsv[s.sv0 + b]; });
+          const double floor = 64.0 * 2.220446049250313e-16 * hsv[s.sv0 + ord[0]];
+          int64_t k = 0;
+          while (k < s.n && hsv[s.sv0 + ord[k]] > floor) ++k;
+          if (k < 2) continue;
+          const Bucket &B = bk[s.bucket];
+          const int nblk = (int)((k + 15) / 16), nbp = nblk + (nblk & 1);
+          const int job = (int)jobs.size();
+          jobs.push_back(SvdJacB{B.rbase + (int64_t)s.slot * B.b * B.mp, B.mp, s.m,
+                                 B.gbase + (int64_t)s.slot * B.b * B.b + (B.b - s.n) * B.b, B.b, k,
+                                 (int64_t)idx.size(), nblk, 0});
+          job_sec.push_back(si);
+          idx.insert(idx.end(), ord.begin(), ord.begin() + k);
+          for (int pi = 0; pi < nbp / 2; ++pi) {
+            pairs.push_back(job);
+            pairs.push_back(pi);
+          }
+          maxnbp = std::max(maxnbp, nbp);
+        }
+        if (jobs.empty()) break;
+        upload(c, dj, jobs);
+        upload(c, dpairs, pairs);
+        upload(c, didx, idx);
+        dflag.ensure(jobs.size() * sizeof(int32_t));
         QLG_HIP(hipMemsetAsync(dflag.p, 0, jobs.size() * sizeof(int32_t), stream));
         for (int round = 0; round < maxnbp - 1; ++round)
           svd_jacobi_round(stream, dj.as<SvdJacB>(), dpairs.as<int32_t>(), (int64_t)pairs.size() / 2,
                            didx.as<int32_t>(), round, R.as<double>(), G.as<double>(), dflag.as<int32_t>(), ed);
         download<int32_t>(c, hflag, dflag.p, jobs.size());
-        bool any = false;
-        for (int32_t f : hflag) any |= f != 0;
-        if (debug) std::fprintf(stderr, "[qlg svd] Jacobi pass %d sweep %d: %zu jobs, rotated %d\n", pass, sweep,
-                                jobs.size(), (int)any);
+        std::vector<int> still;
+        for (size_t q = 0; q < jobs.size(); ++q)
+          if (hflag[q] != 0) still.push_back(job_sec[q]);
+        if (debug) std::fprintf(stderr, "[qlg svd] Jacobi pass %d sweep %d: %zu jobs, %zu still rotating\n", pass,
+                                sweep, jobs.size(), still.size());
         ++jac_sweeps;
-        if (!any) break;
+        active.swap(still);
+        if (!active.empty()) {   // the next sweep's ordering
+          svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
+          download<double>(c, hsv, sv.p, (size_t)nsv);
+        }
       }
       svd_row_norms(stream, dmats.as<SvdMat>(), (int64_t)rm.size(), rowmax, R.as<double>(), sv.as<double>(), ed);
       download<double>(c, hsv, sv.p, (size_t)nsv);

@@ -201,6 +201,23 @@ __global__ __launch_bounds__(VT) void gather_rows_kernel(const SvdRows *__restri
   }
 }
 
+// the inverse of gather_rows: dst row perm[perm + i] (leading dimension dld)
+// = src row i (leading dimension sld), scaled by f[perm + i]
+__global__ __launch_bounds__(VT) void scatter_rows_kernel(const SvdRows *__restrict__ jobs,
+                                                          const int32_t *__restrict__ perm,
+                                                          const double *__restrict__ f, const double *__restrict__ src,
+                                                          double *__restrict__ dst, int ed) {
+  const SvdRows j = jobs[blockIdx.x];
+  const int64_t tot = j.k * j.cols;
+  for (int64_t e = first_el(); e < tot; e += el_stride()) {
+    const int64_t i = e / j.cols, c = e % j.cols;
+    const int64_t dr = perm[j.perm + i];
+    const double fi = f ? f[j.perm + i] : 1.0;
+    for (int q = 0; q < ed; ++q)
+      dst[(j.dst + dr * j.dld + c) * ed + q] = src[(j.src + i * j.sld + c) * ed + q] * fi;
+  }
+}
+
 // Newton-Schulz deviation: max over (i, j) of |A_ij - mask_i delta_ij| for each
 // k x k matrix, per workgroup share: dev[i * gy + y] (the caller takes the max over y)
 __global__ __launch_bounds__(VT) void ns_dev_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ a,
@@ -232,14 +249,18 @@ __global__ __launch_bounds__(VT) void ns_update_kernel(const SvdMat *__restrict_
 
 // Rows of a k x cols matrix whose mask is 0 (zero singular values) are replaced,
 // one after another, by unit vectors orthogonal to every other row: a start
-// vector (unit vector on a cycling coordinate) is orthogonalised twice by
-// classical Gram-Schmidt against all rows with mask 1 or already completed,
-// then normalised.  One workgroup per matrix.
+// vector (unit vector on a cycling coordinate) is orthogonalised twice
+// against all rows with mask 1 or already completed, then normalised.  Each
+// pass takes the rows in chunks of CK: the chunk's coefficients <r_i, r_z>
+// are formed in parallel (one wave per row, lanes over the columns) and then
+// subtracted at once (block Gram-Schmidt).  One workgroup per matrix.
 __global__ __launch_bounds__(VT) void complete_rows_kernel(const SvdMat *__restrict__ mats, double *__restrict__ p,
                                                            double *__restrict__ mask, int ed) {
+  constexpr int CK = 512;
   const SvdMat m = mats[blockIdx.x];
-  __shared__ double cf[2];
+  __shared__ double cf[2 * CK];
   const int64_t k = m.rows, n = m.cols;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int64_t start = 0;
   for (int64_t z = 0; z < k; ++z) {
     if (mask[m.aux + z] != 0.0) continue;
@@ -251,26 +272,48 @@ __global__ __launch_bounds__(VT) void complete_rows_kernel(const SvdMat *__restr
       if (threadIdx.x == 0) rz[(start % n) * ed] = 1.0;
       __syncthreads();
       for (int pass = 0; pass < 2; ++pass)
-        for (int64_t i = 0; i < k; ++i) {
-          if (i == z || mask[m.aux + i] == 0.0) continue;
-          const double *ri = p + (m.off + i * m.ld) * ed;
-          double sr = 0.0, si = 0.0;   // <r_i, r_z> = sum conj(r_i) r_z
-          for (int64_t c = threadIdx.x; c < n; c += VT) {
-            if (ed == 2) {
-              sr += ri[2 * c] * rz[2 * c] + ri[2 * c + 1] * rz[2 * c + 1];
-              si += ri[2 * c] * rz[2 * c + 1] - ri[2 * c + 1] * rz[2 * c];
-            } else {
-              sr += ri[c] * rz[c];
+        for (int64_t i0 = 0; i0 < k; i0 += CK) {
+          const int nc = (int)min<int64_t>(CK, k - i0);
+          for (int ii = wave; ii < nc; ii += VT / 64) {
+            const int64_t i = i0 + ii;
+            double sr = 0.0, si = 0.0;   // <r_i, r_z> = sum conj(r_i) r_z
+            if (i != z && mask[m.aux + i] != 0.0) {
+              const double *ri = p + (m.off + i * m.ld) * ed;
+              for (int64_t c = lane; c < n; c += 64) {
+                if (ed == 2) {
+                  sr += ri[2 * c] * rz[2 * c] + ri[2 * c + 1] * rz[2 * c + 1];
+                  si += ri[2 * c] * rz[2 * c + 1] - ri[2 * c + 1] * rz[2 * c];
+                } else {
+                  sr += ri[c] * rz[c];
+                }
+              }
+              sr = wave_sum(sr);
+              si = wave_sum(si);
+            }
+            if (lane == 0) {
+              cf[2 * ii] = sr;
+              cf[2 * ii + 1] = si;
             }
           }
-          sr = block_sum(sr);
-          si = block_sum(si);
+          __syncthreads();
           for (int64_t c = threadIdx.x; c < n; c += VT) {
+            double ar = 0.0, ai = 0.0;
+            for (int ii = 0; ii < nc; ++ii) {
+              const double cr = cf[2 * ii], ci = cf[2 * ii + 1];
+              if (cr == 0.0 && ci == 0.0) continue;
+              const double *ri = p + (m.off + (i0 + ii) * m.ld) * ed;
+              if (ed == 2) {
+                ar += cr * ri[2 * c] - ci * ri[2 * c + 1];
+                ai += cr * ri[2 * c + 1] + ci * ri[2 * c];
+              } else {
+                ar += cr * ri[c];
+              }
+            }
             if (ed == 2) {
-              rz[2 * c] -= sr * ri[2 * c] - si * ri[2 * c + 1];
-              rz[2 * c + 1] -= sr * ri[2 * c + 1] + si * ri[2 * c];
+              rz[2 * c] -= ar;
+              rz[2 * c + 1] -= ai;
             } else {
-              rz[c] -= sr * ri[c];
+              rz[c] -= ar;
             }
           }
           __syncthreads();
@@ -592,6 +635,10 @@ void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxrow
                    int ed) {
   const int64_t gy = std::max<int64_t>(1, std::min<int64_t>((maxrows + VT / 64 - 1) / (VT / 64), 1024));
   QLG_LAUNCH2(row_norms_kernel, n, gy, mats, x, out, ed);
+}
+void svd_scatter_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
+                      const double *f, const double *src, double *dst, int ed) {
+  QLG_LAUNCH2(scatter_rows_kernel, n, svd_grid_y(maxel), jobs, perm, f, src, dst, ed);
 }
 void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
                      const double *f, const double *src, double *dst, int ed) {
