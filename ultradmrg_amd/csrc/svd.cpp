@@ -353,6 +353,8 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
     std::vector<int> todo;
     for (size_t i = 0; i < bk.size(); ++i)
       if (bk[i].b > 1) todo.push_back((int)i);
+    std::vector<int32_t> nan_flag;   // the last check: one flag per matrix of nan_bucket, in order
+    std::vector<int> nan_bucket;
     DevBuf G0;
     if (cplx && !todo.empty()) {
       G0.ensure((size_t)std::max<int64_t>(gtot, 1) * ed * 8);
@@ -404,39 +406,48 @@ SvdResult *block_svd(Ctx &c, const qlg_bsten *desc, int ldims, const int32_t *ld
         svd_reflect_rows(stream, dmats2.as<SvdMat>() + pos, nb, G.as<double>(), du.as<double>());
         pos += 2 * nb;
       }
-      // NaN check, per bucket
-      std::vector<int> left;
+      // NaN check on the device, one flag per matrix
+      std::vector<SvdMat> nm;
+      int64_t nmax = 0;
       for (int i : todo) {
         const Bucket &B = bk[i];
-        std::vector<double> h;
-        download<double>(c, h, G.as<double>() + B.gbase * ed, (size_t)B.secs.size() * B.b * B.b * ed);
+        for (size_t j = 0; j < B.secs.size(); ++j)
+          nm.push_back(SvdMat{B.gbase + (int64_t)j * B.b * B.b, B.b, B.b, B.b, (int64_t)nm.size()});
+        nmax = std::max(nmax, B.b * B.b * ed);
+      }
+      DevBuf dnm, dnf;
+      upload(c, dnm, nm);
+      dnf.ensure(std::max<size_t>(1, nm.size()) * sizeof(int32_t));
+      QLG_HIP(hipMemsetAsync(dnf.p, 0, std::max<size_t>(1, nm.size()) * sizeof(int32_t), stream));
+      svd_nan_flags(stream, dnm.as<SvdMat>(), (int64_t)nm.size(), nmax, G.as<double>(), dnf.as<int32_t>(), ed);
+      download<int32_t>(c, nan_flag, dnf.p, nm.size());
+      nan_bucket.clear();
+      std::vector<int> left;
+      size_t fp = 0;
+      for (int i : todo) {
         bool bad = false;
-        for (double v : h)
-          if (std::isnan(v)) {
-            bad = true;
-            break;
-          }
+        for (size_t j = 0; j < bk[i].secs.size(); ++j, ++fp) bad |= nan_flag[fp] != 0;
         if (bad) left.push_back(i);
+        nan_bucket.push_back(i);
       }
       todo.swap(left);
     }
     // matrices still with NaN vectors: E^H row (b - n + i) = e_i, so R = X
-    for (int i : todo) {
-      const Bucket &B = bk[i];
-      std::vector<double> h;
-      download<double>(c, h, G.as<double>() + B.gbase * ed, (size_t)B.secs.size() * B.b * B.b * ed);
-      for (size_t j = 0; j < B.secs.size(); ++j) {
-        double *m = h.data() + j * B.b * B.b * ed;
-        bool bad = false;
-        for (int64_t e = 0; e < B.b * B.b * ed && !bad; ++e) bad = std::isnan(m[e]);
-        if (!bad) continue;
-        const Sector &sj = sec[B.secs[j]];
-        std::fill(m, m + B.b * B.b * ed, 0.0);
-        for (int64_t r = 0; r < sj.n; ++r) m[((B.b - sj.n + r) * B.b + r) * ed] = 1.0;
-        forced[B.secs[j]] = 1;
+    {
+      size_t pos = 0;
+      for (int i : nan_bucket) {
+        const Bucket &B = bk[i];
+        for (size_t j = 0; j < B.secs.size(); ++j, ++pos) {
+          if (std::find(todo.begin(), todo.end(), i) == todo.end() || nan_flag[pos] == 0) continue;
+          const Sector &sj = sec[B.secs[j]];
+          std::vector<double> m((size_t)B.b * B.b * ed, 0.0);
+          for (int64_t r = 0; r < sj.n; ++r) m[((B.b - sj.n + r) * B.b + r) * ed] = 1.0;
+          QLG_HIP(hipMemcpyAsync(G.as<double>() + (B.gbase + (int64_t)j * B.b * B.b) * ed, m.data(), m.size() * 8,
+                                 hipMemcpyHostToDevice, stream));
+          QLG_HIP(hipStreamSynchronize(stream));
+          forced[B.secs[j]] = 1;
+        }
       }
-      QLG_HIP(hipMemcpyAsync(G.as<double>() + B.gbase * ed, h.data(), h.size() * 8, hipMemcpyHostToDevice, stream));
-      QLG_HIP(hipStreamSynchronize(stream));
     }
     for (Bucket &B : bk)   // the b = 1 bucket: E^H = [1]
       if (B.b == 1) {

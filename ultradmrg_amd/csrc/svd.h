@@ -52,6 +52,9 @@ void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxrow
                    int ed);
 void svd_gather_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
                      const double *f, const double *src, double *dst, int ed);
+// flag[mats[i].aux] = 1 if the rows x rows matrix at mats[i].off holds a NaN (flags cleared by the caller)
+void svd_nan_flags(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, const double *g, int32_t *flag,
+                   int ed);
 // dst row perm[perm + i] = f[perm + i] * src row i (SvdRows: src / sld the packed rows, dst / dld the target)
 void svd_scatter_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
                       const double *f, const double *src, double *dst, int ed);

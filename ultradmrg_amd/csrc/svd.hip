@@ -201,6 +201,17 @@ __global__ __launch_bounds__(VT) void gather_rows_kernel(const SvdRows *__restri
   }
 }
 
+// flag[m.aux] = 1 if matrix m (rows x rows at off) holds a NaN; flags are
+// cleared by the caller, workgroups of one matrix may store the same 1
+__global__ __launch_bounds__(VT) void nan_flags_kernel(const SvdMat *__restrict__ mats, const double *__restrict__ g,
+                                                       int32_t *__restrict__ flag, int ed) {
+  const SvdMat m = mats[blockIdx.x];
+  const int64_t tot = m.rows * m.rows * ed;
+  bool bad = false;
+  for (int64_t e = first_el(); e < tot; e += el_stride()) bad |= isnan(g[m.off * ed + e]);
+  if (__any(bad) && (threadIdx.x & 63) == 0) flag[m.aux] = 1;
+}
+
 // the inverse of gather_rows: dst row perm[perm + i] (leading dimension dld)
 // = src row i (leading dimension sld), scaled by f[perm + i]
 __global__ __launch_bounds__(VT) void scatter_rows_kernel(const SvdRows *__restrict__ jobs,
@@ -635,6 +646,10 @@ void svd_row_norms(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxrow
                    int ed) {
   const int64_t gy = std::max<int64_t>(1, std::min<int64_t>((maxrows + VT / 64 - 1) / (VT / 64), 1024));
   QLG_LAUNCH2(row_norms_kernel, n, gy, mats, x, out, ed);
+}
+void svd_nan_flags(hipStream_t st, const SvdMat *mats, int64_t n, int64_t maxel, const double *g, int32_t *flag,
+                   int ed) {
+  QLG_LAUNCH2(nan_flags_kernel, n, svd_grid_y(maxel), mats, g, flag, ed);
 }
 void svd_scatter_rows(hipStream_t st, const SvdRows *jobs, int64_t n, int64_t maxel, const int32_t *perm,
                       const double *f, const double *src, double *dst, int ed) {
