@@ -51,9 +51,9 @@ import time
 # The block SVD runs its eigen-decompositions on four streams beside the
 # solver's own; with HIP's default of four hardware queues per process some
 # of them share a queue and serialise (the largest bucket's lane then waits
-# behind another's launches).  Eight queues keep them apart: 0.25-0.4 s less
-# SVD time per D=2000 sweep.  Read by HIP at its first call; a value already
-# in the environment wins.
+# behind another's launches).  Eight queues keep them apart (0.25-0.4 s less
+# SVD time per D=2000 sweep in a back-to-back comparison on one box).  Read
+# by HIP at its first call; a value already in the environment wins.
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
