@@ -30,6 +30,11 @@
 namespace qlg {
 
 namespace {
+struct PairHash {
+  size_t operator()(const std::pair<int, int> &p) const {
+    return std::hash<uint64_t>()((uint64_t(uint32_t(p.first)) << 32) | uint32_t(p.second));
+  }
+};
 // QLG_VERBOSE=2: host-side phase times of plan construction (stderr)
 struct PhaseTimer {
   bool on;
@@ -101,11 +106,11 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
   // ---- indexes of operator blocks ------------------------------------------
   std::map<int, std::vector<int64_t>> L_by_ket;             // a -> L blocks
   for (size_t b = 0; b < L.blk.size(); ++b) L_by_ket[L.blk[b][2]].push_back(b);
-  std::map<std::pair<int, int>, std::vector<int64_t>> W_by_in[2];   // (wl, s_in) -> W blocks
+  std::unordered_map<std::pair<int, int>, std::vector<int64_t>, PairHash> W_by_in[2];   // (wl, s_in) -> W blocks
   for (int k = 0; k < nphys; ++k)
     for (size_t b = 0; b < p.W[k].blk.size(); ++b)
       W_by_in[k][{p.W[k].blk[b][0], p.W[k].blk[b][1]}].push_back(b);
-  std::map<std::pair<int, int>, std::vector<int64_t>> R_by_zb;    // (z, b) -> R blocks
+  std::unordered_map<std::pair<int, int>, std::vector<int64_t>, PairHash> R_by_zb;    // (z, b) -> R blocks
   std::map<int, std::vector<int64_t>> R_by_bra;                  // b' -> R blocks
   for (size_t b = 0; b < R.blk.size(); ++b) {
     R_by_zb[{R.blk[b][1], R.blk[b][2]}].push_back(b);
@@ -359,7 +364,9 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     int64_t base = 0, rows = 0, K = 0, ld = 0;
   };
   std::map<int, YPanel> ypn;
-  std::map<std::array<int64_t, 3>, int64_t> yrow;   // (vec block, comp0, comp1) -> first row
+  // first Y row of (vec block vb, components c0, c1):
+  // yrow0[vb] + c0 * ystep0[vb] + c1 * ystep1[vb]
+  std::vector<int64_t> yrow0(p.vec.blk.size(), -1), ystep0(p.vec.blk.size(), 0), ystep1(p.vec.blk.size(), 0);
   {
     std::map<int, std::vector<int64_t>> V_by_right;
     for (size_t vb = 0; vb < p.vec.blk.size(); ++vb) V_by_right[p.vec.blk[vb][bax]].push_back(vb);
@@ -378,9 +385,11 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
         const int ap = o[0];
         const int64_t Dap = L.ax[0].deg[ap];
         const int64_t d0 = p.W[0].ax[2].deg[o[1]], d1 = nphys == 2 ? p.W[1].ax[2].deg[o[2]] : 1;
+        yrow0[vb] = Y.rows;
+        ystep0[vb] = d1 * Dap;
+        ystep1[vb] = Dap;
         for (int c0 = 0; c0 < d0; ++c0)
           for (int c1 = 0; c1 < d1; ++c1) {
-            yrow[{vb, c0, c1}] = Y.rows;
             const int64_t col0 = fz[ap].base + fcol[vb] + (c0 * d1 + c1) * Dbp;
             for (int64_t r = 0; r < Dap; ++r) s3.crows.push_back(col0 + r * fz[ap].N);
             Y.rows += Dap;
@@ -589,11 +598,17 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       ob[1] = o[0];
       if (nphys == 2) ob[2] = o[2];
       ob[bax] = bp;
-      const int64_t vb = p.vec.find[ob];
-      const YPanel &Y = ypn[bp];
-      const int64_t row0 = yrow[{vb, o[1], o[3]}];
+      const auto vit = p.vec.find.find(ob);
+      const auto yit = ypn.find(bp);
+      const auto rit = rpn.find(bp);
+      QLG_CHECK(vit != p.vec.find.end() && yit != ypn.end() && rit != rpn.end(), "planner: channel output lost");
+      const int64_t vb = vit->second;
+      const auto cit = rit->second.col.find({o[4], c.b});
+      QLG_CHECK(yrow0[vb] >= 0 && cit != rit->second.col.end(), "planner: channel output lost");
+      const YPanel &Y = yit->second;
+      const int64_t row0 = yrow0[vb] + (int64_t)o[1] * ystep0[vb] + (int64_t)o[3] * ystep1[vb];
       RowDesc d;
-      d.off = Y.base + row0 * Y.ld + rpn[bp].col[{o[4], c.b}] + (int64_t)o[5] * Db;
+      d.off = Y.base + row0 * Y.ld + cit->second + (int64_t)o[5] * Db;
       d.rs = Y.ld;
       s2.rows.push_back(d);
     }
