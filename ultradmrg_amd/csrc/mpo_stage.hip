@@ -12,9 +12,10 @@
 // columns, the C chunk (BM x KC) is staged in LDS, and each lane loads the
 // X values of its MFMA B fragments for the whole K chunk straight into
 // registers before the first MFMA.  Raw buffer loads return zero outside
-// the channel (tile_io.h).  The first kRowWin input-row descriptors of a
-// channel are staged in LDS; further ones (channels with many MPO/physical
-// components) are read from memory.
+// the channel (tile_io.h).  Input-row descriptors are staged in LDS,
+// kRowWin at a time (channels with many MPO/physical components take several
+// windows), and the C and X loads of a chunk are all issued before the first
+// wait, so a chunk costs one memory latency.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -70,7 +71,12 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   static_assert(FM >= 1 && FN >= 1, "tile too small");
   __shared__ T As[NP][KC][BM + 1];
   __shared__ RowDesc rin[kRowWin];
-  __shared__ RowDesc rout[BM];
+  __shared__ RowDesc rout[256];   // BM used; one slot per thread, written unconditionally
+
+  static_assert(kRowWin == 256 && BM <= 256, "one descriptor per thread");
+  static_assert(kRowWin % KC == 0, "a descriptor window holds whole K chunks");
+  constexpr int CE = BM * KC / 256;         // C-chunk elements per thread
+  static_assert(CE * 256 == BM * KC, "C chunk splits evenly over the threads");
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -83,8 +89,21 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   const T *Cm = C + tk.c_off * NP;
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(X);
   const RowDesc *gin = rows + tk.in_rows;
-  for (int i = tid; i < min(K, kRowWin); i += 256) rin[i] = gin[i];
-  for (int i = tid; i < BM; i += 256) rout[i] = rows[tk.out_rows + min(tl.m0 + i, M - 1)];
+  // Input-row descriptors are only ever read from LDS: rows [w, w + kRowWin)
+  // of the channel sit in rin while chunk kc (w = kc rounded down to
+  // kRowWin) is loaded.  (A descriptor read that could come from either LDS
+  // or memory compiles to a generic load, which waits for every X load
+  // issued before it: one memory latency per k-step.)  Both staging loads
+  // are issued before either LDS write.
+  {
+    // clamped indexes: every thread loads an existing descriptor (K, M >= 1)
+    const RowDesc din = gin[min(tid, K - 1)];
+    const RowDesc dout = rows[tk.out_rows + min(tl.m0 + min(tid, BM - 1), M - 1)];
+    // unconditional writes: a conditional one lets the compiler sink its
+    // load into the branch and wait for each load in turn
+    rin[tid] = din;
+    rout[tid] = dout;
+  }
 
   // this lane's columns: (a', b) positions of its B fragments
   int apj[FN], bj[FN];
@@ -106,27 +125,35 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
       for (int j = 0; j < FN; ++j) acc[p][i][j] = acc_t{0, 0, 0, 0};
 
   for (int kc = 0; kc < K; kc += KC) {
-    __syncthreads();  // descriptors staged / previous chunk of As consumed
-    for (int e = tid; e < BM * KC; e += 256) {
+    __syncthreads();  // descriptors staged / previous chunk's As and rin reads done
+    const int win = kc & ~(kRowWin - 1);
+    if (kc > 0 && kc == win) {  // channels with more than kRowWin input rows: the next window
+      if (tid < min(K - win, kRowWin)) rin[tid] = gin[win + tid];
+      __syncthreads();
+    }
+    // the C chunk (rows m0.., columns kc..), loaded into registers first and
+    // written to LDS after the X loads below are issued, so that the chunk
+    // waits on memory once
+    T cv[NP][CE];
+#pragma unroll
+    for (int q = 0; q < CE; ++q) {
+      const int e = tid + 256 * q;
       const int m = e / KC, k = e % KC;
       const int gm = tl.m0 + m, gk = kc + k;
       const bool ok = gm < M && gk < K;
       const int64_t ix = ok ? (int64_t)gm * K + gk : 0;
       if constexpr (CPLX) {
-        T vr, vi;
         if constexpr (sizeof(T) == 8) {
           const double2 v = reinterpret_cast<const double2 *>(Cm)[ix];
-          vr = v.x;
-          vi = v.y;
+          cv[0][q] = ok ? v.x : T(0);
+          cv[1][q] = ok ? v.y : T(0);
         } else {
           const float2 v = reinterpret_cast<const float2 *>(Cm)[ix];
-          vr = v.x;
-          vi = v.y;
+          cv[0][q] = ok ? v.x : T(0);
+          cv[1][q] = ok ? v.y : T(0);
         }
-        As[0][k][m] = ok ? vr : T(0);
-        As[1][k][m] = ok ? vi : T(0);
       } else {
-        As[0][k][m] = ok ? Cm[ix] : T(0);
+        cv[0][q] = ok ? Cm[ix] : T(0);
       }
     }
     // B fragments of the whole chunk: lane holds X[row kc + 4 s + lane / 16][its column j].
@@ -139,7 +166,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
       if (kc + 4 * st >= K) break;
       const int gk = kc + 4 * st + (lane >> 4);
       const bool kok = gk < K;
-      const RowDesc rd = K <= kRowWin ? rin[kok ? gk : 0] : gin[kok ? gk : 0];
+      const RowDesc rd = rin[kok ? gk - win : 0];
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const uint32_t el = (kok && okj[j]) ? (uint32_t)((int)rd.off + apj[j] * (int)rd.rs + bj[j]) : kOob;
@@ -155,6 +182,12 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
           rb[0][st][j] = MF::load(xr, el);
         }
       }
+    }
+#pragma unroll
+    for (int q = 0; q < CE; ++q) {
+      const int e = tid + 256 * q;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) As[p][e % KC][e / KC] = cv[p][q];
     }
     __syncthreads();  // C chunk staged
 #pragma unroll
