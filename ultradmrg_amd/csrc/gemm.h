@@ -62,6 +62,15 @@ struct ChanTask {
 };
 
 
+// An MPO-stage tile with its channel inlined, so that a workgroup's first
+// memory access is one scalar load instead of tile -> channel; m0 < 0 marks
+// an empty launch slot.
+struct ChanTile {
+  ChanTask c;
+  int32_t m0, n0;
+  int32_t pad[2];
+};
+
 struct RowDesc {
   int64_t off;   // element offset of position (0, 0)
   int64_t rs;    // element stride between consecutive a'

@@ -908,9 +908,8 @@ void Plan::launch_stage(int i, const Slots &s) {
   Stage &g = st[i];
   const int out_planes = planar() ? (precision == QLG_PREC_BF16X6 ? 3 : 2) : 0;
   if (i == 1) {
-    launch_mpo_stage(g.bm, g.bn, cplx, reduced, out_planes, y_elems, g.d_tiles.as<GemmTile>(),
-                     g.d_chan.as<ChanTask>(), g.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], s.skip,
-                     (int)g.tiles.size(), stream);
+    launch_mpo_stage(g.bm, g.bn, cplx, reduced, out_planes, y_elems, g.d_ctiles.as<ChanTile>(),
+                     g.d_rows.as<RowDesc>(), s.p[SLOT_C], X.p, s.p[SLOT_Y], s.skip, (int)g.ctiles.size(), stream);
     return;
   }
   // the parts write their panels through SLOT_W: it must name this stage's workspace as uploaded

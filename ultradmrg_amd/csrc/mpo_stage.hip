@@ -28,7 +28,12 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 namespace {
-constexpr int KC = 32;        // K chunk held in registers per round
+// K chunk held in registers per round.  FP64: 16 rows, which keeps the
+// 32x128 kernel at 6 waves per SIMD (32 rows: 4); the channels' K_in is ~16 on
+// average, so the extra chunks of the longer channels cost less than the
+// lost occupancy did.  FP32 kernels already run at 7-8 waves with 32.
+template <typename T>
+constexpr int kChunk = sizeof(T) == 8 ? 16 : 32;
 constexpr int kRowWin = 256;
 
 template <typename T>
@@ -57,8 +62,7 @@ struct Mfma<float> {    // v_mfma_f32_16x16x4: D lane l, reg r -> row 4 (l / 16)
 // FP32 result x, h = bf16(x), m = bf16(x - h), l = bf16(x - h - m), plane q
 // at yps elements after plane q - 1 (the operand of the pre-split stage 3).
 template <int BM, int BN, bool CPLX, typename T, int OUTP = 0>
-__global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restrict__ tiles,
-                                                        const ChanTask *__restrict__ tasks,
+__global__ __launch_bounds__(256) void mpo_stage_kernel(const ChanTile *__restrict__ tiles,
                                                         const RowDesc *__restrict__ rows, const T *__restrict__ C,
                                                         const T *__restrict__ X, T *__restrict__ Y,
                                                         const int *__restrict__ skip, int64_t yps, int ntiles) {
@@ -67,6 +71,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   constexpr int WN = BN / 4;                // columns per wave
   constexpr int FM = BM / 16, FN = WN / 16;
   constexpr int NP = CPLX ? 2 : 1;
+  constexpr int KC = kChunk<T>;
   constexpr int KS = KC / 4;                // MFMA k-steps per chunk
   static_assert(FM >= 1 && FN >= 1, "tile too small");
   __shared__ T As[NP][KC][BM + 1];
@@ -81,10 +86,10 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const GemmTile *__restri
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   if ((int)blockIdx.x >= ntiles) return;
+  const ChanTile tl = tiles[blockIdx.x];   // issued before the flag load, waited for together
   if (skip && *skip) return;   // the Krylov solve has stopped
-  const GemmTile tl = tiles[blockIdx.x];
-  if (tl.task < 0) return;
-  const ChanTask tk = tasks[tl.task];
+  if (tl.m0 < 0) return;
+  const ChanTask &tk = tl.c;
   const int M = tk.kout, N = tk.P, K = tk.kin;
   const T *Cm = C + tk.c_off * NP;
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(X);
@@ -323,9 +328,9 @@ void launch_trans_copy(bool cplx, const TransDesc *d, int n, const double *src, 
   QLG_HIP(hipGetLastError());
 }
 
-void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const GemmTile *tiles,
-                      const ChanTask *tasks, const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip,
-                      int ntiles, hipStream_t st) {
+void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const ChanTile *tiles,
+                      const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip, int ntiles,
+                      hipStream_t st) {
   if (ntiles <= 0) return;
   dim3 grid(ntiles), block(256);
   const double *Cd = static_cast<const double *>(C), *Xd = static_cast<const double *>(X);
@@ -335,24 +340,24 @@ void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64
 #define QLG_MPO(M_, N_)                                                                                       \
   if (bm == M_ && bn == N_) {                                                                                 \
     if (out_planes == 3 && f32 && !cplx)                                                                      \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float, 3>), grid, block, 0, st, tiles, tasks, rows,  \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float, 3>), grid, block, 0, st, tiles, rows,  \
                          Cf, Xf, Yf, skip, yps, ntiles);                                                      \
     else if (out_planes == 2 && f32 && !cplx)                                                                 \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float, 2>), grid, block, 0, st, tiles, tasks, rows,  \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float, 2>), grid, block, 0, st, tiles, rows,  \
                          Cf, Xf, Yf, skip, yps, ntiles);                                                      \
     else if (out_planes != 0)                                                                                 \
       throw Error(QLG_ERR_UNSUPPORTED, "MPO stage: piece-plane output is for real FP32 data");               \
     else if (f32 && cplx)                                                                                     \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, float>), grid, block, 0, st, tiles, tasks, rows, Cf,  \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, float>), grid, block, 0, st, tiles, rows, Cf,  \
                          Xf, Yf, skip, 0, ntiles);                                                                     \
     else if (f32)                                                                                             \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float>), grid, block, 0, st, tiles, tasks, rows, Cf, \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, float>), grid, block, 0, st, tiles, rows, Cf, \
                          Xf, Yf, skip, 0, ntiles);                                                                     \
     else if (cplx)                                                                                            \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, double>), grid, block, 0, st, tiles, tasks, rows, Cd, \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, true, double>), grid, block, 0, st, tiles, rows, Cd, \
                          Xd, Yd, skip, 0, ntiles);                                                                     \
     else                                                                                                      \
-      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, double>), grid, block, 0, st, tiles, tasks, rows,   \
+      hipLaunchKernelGGL((mpo_stage_kernel<M_, N_, false, double>), grid, block, 0, st, tiles, rows,   \
                          Cd, Xd, Yd, skip, 0, ntiles);                                                                 \
     QLG_HIP(hipGetLastError());                                                                               \
     return;                                                                                                   \

@@ -289,6 +289,22 @@ void Stage::upload(hipStream_t st) {
     QLG_HIP(hipMemcpyAsync(d_chan.p, chan.data(), chan.size() * sizeof(ChanTask), hipMemcpyHostToDevice, st));
     if (!rows.empty())
       QLG_HIP(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(RowDesc), hipMemcpyHostToDevice, st));
+    ctiles.assign(tiles.size(), ChanTile{});
+    for (size_t i = 0; i < tiles.size(); ++i) {
+      const GemmTile &t = tiles[i];
+      ChanTile &o = ctiles[i];
+      if (t.task < 0) {
+        o.m0 = -1;
+        continue;
+      }
+      QLG_CHECK((size_t)t.task < chan.size(), "MPO-stage tile: channel index out of range");
+      o.c = chan[t.task];
+      o.m0 = t.m0;
+      o.n0 = t.n0;
+    }
+    d_ctiles.ensure(std::max<size_t>(1, ctiles.size()) * sizeof(ChanTile));
+    if (!ctiles.empty())
+      QLG_HIP(hipMemcpyAsync(d_ctiles.p, ctiles.data(), ctiles.size() * sizeof(ChanTile), hipMemcpyHostToDevice, st));
   }
   if (!crows.empty()) {
     d_crows.ensure(crows.size() * sizeof(int64_t));

@@ -22,7 +22,8 @@ struct Stage {
   std::vector<int64_t> crows;      // output-row address table (tasks with c_rowtab)
   std::vector<ChanTask> chan;      // MPO-stage channels (instead of tasks)
   std::vector<RowDesc> rows;       // their gathered / scattered rows
-  DevBuf d_tiles, d_tasks, d_segs, d_crows, d_chan, d_rows;
+  std::vector<ChanTile> ctiles;    // MPO stage: tiles with their channels inlined (upload)
+  DevBuf d_tiles, d_tasks, d_segs, d_crows, d_chan, d_rows, d_ctiles;
   int bm = 64, bn = 64;
   double flops = 0;      // useful 2*M*N*K issued by this stage
   double algo_flops = 0; // this stage's share of the reference schedule's FLOPs
@@ -158,9 +159,9 @@ std::vector<double> dense_copy(const Ten &t, hipStream_t st, int ed);
 int64_t raw_elems(const Ten &t);
 std::vector<double> densify(const Ten &t, const double *host_raw, int ed);
 void make_tiles_lpt(std::vector<GemmTile> &tiles, const std::vector<std::array<int64_t, 3>> &mnk, int bm, int bn);
-void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const GemmTile *tiles,
-                      const ChanTask *tasks, const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip,
-                      int ntiles, hipStream_t st);
+void launch_mpo_stage(int bm, int bn, bool cplx, bool f32, int out_planes, int64_t yps, const ChanTile *tiles,
+                      const RowDesc *rows, const void *C, const void *X, void *Y, const int *skip, int ntiles,
+                      hipStream_t st);
 // split-BF16 on pre-split operands (gemm_mixed.hip): stage 1 (A planes, B the
 // FP64 vector) or stage 3 (A and B planes, FP64 output); FP64 -> 3 bf16 planes
 void launch_grouped_gemm_planar(int mode, int bm, int bn, bool stage3, const GemmTile *tiles, const GemmTask *tasks,
