@@ -26,6 +26,7 @@
 #include <cfloat>
 
 #include "krylov_dev.h"
+#include "tile_io.h"
 #include "vecops.h"
 
 namespace qlg {
@@ -311,14 +312,14 @@ __global__ __launch_bounds__(VT) void multidot_kernel(int64_t n, const double *c
   constexpr int NC = CPLX ? 2 : 1;
   __shared__ double sh[NC][VT / 64];
   const int j = blockIdx.y;
-  const double *v = bases[j];
+  const auto *v = global_ptr(bases[j]);
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * VT;
   for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += stride) {
     if constexpr (CPLX) {
-      const double2 x = reinterpret_cast<const double2 *>(v)[i];
+      const double2 x = global_c128(bases[j], i);
       const double2 y = reinterpret_cast<const double2 *>(g)[i];
       acc[0] += x.x * y.x + x.y * y.y;
       acc[1] += x.x * y.y - x.y * y.x;
@@ -364,8 +365,9 @@ __global__ __launch_bounds__(VT) void reorth_update_kernel(int64_t n, const doub
   for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += stride) {
     if constexpr (CPLX) {
       double2 y = reinterpret_cast<double2 *>(g)[i];
+#pragma unroll 8
       for (int j = 0; j < k; ++j) {
-        const double2 x = reinterpret_cast<const double2 *>(bases[j])[i];
+        const double2 x = global_c128(bases[j], i);
         const double cr = c[2 * j], ci = c[2 * j + 1];
         y.x -= cr * x.x - ci * x.y;
         y.y -= cr * x.y + ci * x.x;
@@ -374,7 +376,8 @@ __global__ __launch_bounds__(VT) void reorth_update_kernel(int64_t n, const doub
       acc[0] += y.x * y.x + y.y * y.y;
     } else {
       double y = g[i];
-      for (int j = 0; j < k; ++j) y -= c[j] * bases[j][i];
+#pragma unroll 8
+      for (int j = 0; j < k; ++j) y -= c[j] * global_ptr(bases[j])[i];
       g[i] = y;
       acc[0] += y * y;
     }

@@ -20,6 +20,23 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void *base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)kOob, kRsrcWord3);
 }
 
+// A pointer read from a pointer table has no known address space, so loads
+// through it compile to generic (flat) loads, which count on the scalar-memory
+// counter too: the wait for the next table entry then also waits for every
+// element load issued before it.  The tables hold device allocations, so the
+// pointer is global memory.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *global_ptr(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+
+// complex element i (re, im) through such a pointer
+__device__ inline double2 global_c128(const double *p, int64_t i) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 v = global_ptr(reinterpret_cast<const d2 *>(p))[i];
+  return double2{v.x, v.y};
+}
+
 __device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }

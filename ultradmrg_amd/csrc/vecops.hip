@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "tile_io.h"
 #include "vecops.h"
 
 namespace qlg {
@@ -133,7 +134,9 @@ __global__ __launch_bounds__(VT) void lincomb_kernel(int64_t nd, int k, const do
   const int64_t stride = (int64_t)gridDim.x * VT;
   for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < nd; i += stride) {
     double s = use_beta ? beta * y[i] : 0.0;
-    for (int j = 0; j < k; ++j) s += coefs[j] * bases[j][i];
+    // unrolled so that eight bases' loads are in flight together (the sum keeps its order)
+#pragma unroll 8
+    for (int j = 0; j < k; ++j) s += coefs[j] * global_ptr(bases[j])[i];
     y[i] = s;
   }
 }
@@ -151,8 +154,9 @@ __global__ __launch_bounds__(VT) void zlincomb_kernel(int64_t n, int k, const do
       sr = br * v.x - bi * v.y;
       si = br * v.y + bi * v.x;
     }
+#pragma unroll 8
     for (int j = 0; j < k; ++j) {
-      const double2 v = reinterpret_cast<const double2 *>(bases[j])[i];
+      const double2 v = global_c128(bases[j], i);
       const double cr = coefs[2 * j], ci = coefs[2 * j + 1];
       sr += cr * v.x - ci * v.y;
       si += cr * v.y + ci * v.x;
@@ -337,7 +341,8 @@ __global__ __launch_bounds__(VT) void seg_lincomb_kernel(const SegChunk *__restr
   const SegChunk c = ch[blockIdx.x];
   for (int64_t i = c.begin + threadIdx.x; i < c.end; i += VT) {
     double s = 0.0;
-    for (int j = 0; j < k; ++j) s += coefs[(int64_t)j * nseg + c.seg] * bases[j][i];
+#pragma unroll 8
+    for (int j = 0; j < k; ++j) s += coefs[(int64_t)j * nseg + c.seg] * global_ptr(bases[j])[i];
     y[i] = s;
   }
 }
