@@ -216,7 +216,21 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const ChanTile *__restri
         }
     }
   }
-  // scatter: row m -> rout[m - m0], column n -> (n / Db, n % Db)
+  // scatter: row m -> rout[m - m0], column n -> (n / Db, n % Db).  The
+  // lane's row descriptors are all read from LDS before the first store
+  // (every rout slot holds a valid descriptor): read inside the guarded
+  // store, each would wait for its own LDS round trip.
+  // (row strides are leading dimensions of the Y panels, below 2^31)
+  int64_t ro[FM][4];
+  int32_t rr[FM][4];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const RowDesc d = rout[i * 16 + MF::row(lane, r)];
+      ro[i][r] = d.off;
+      rr[i][r] = (int32_t)d.rs;
+    }
 #pragma unroll
   for (int j = 0; j < FN; ++j) {
     if (!okj[j]) continue;
@@ -226,8 +240,7 @@ __global__ __launch_bounds__(256) void mpo_stage_kernel(const ChanTile *__restri
       for (int r = 0; r < 4; ++r) {
         const int lr = i * 16 + MF::row(lane, r);
         if (tl.m0 + lr < M) {
-          const RowDesc rd = rout[lr];
-          const int64_t ix = rd.off + (int64_t)apj[j] * rd.rs + bj[j];
+          const int64_t ix = ro[i][r] + (int64_t)apj[j] * rr[i][r] + bj[j];
           if constexpr (OUTP > 0) {
             float x = acc[0][i][j][r];
 #pragma unroll

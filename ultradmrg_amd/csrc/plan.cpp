@@ -348,6 +348,8 @@ void validate_stage(const Stage &s, const int64_t *slot_elems) {
       const RowDesc &d = s.rows[c.out_rows + r];
       QLG_CHECK(d.off >= 0 && d.rs >= c.Db && d.off + (na - 1) * d.rs + c.Db - 1 < slot_elems[SLOT_Y],
                 "task list: channel output row outside the second intermediate");
+      // the MPO-stage kernel keeps output row strides in 32 bits
+      QLG_CHECK(d.rs < (int64_t(1) << 31), "task list: channel output row stride beyond 32 bits");
     }
   }
   auto inside = [&](int slot, int64_t first, int64_t last, const char *what) {
