@@ -25,6 +25,7 @@
 #include <unordered_map>
 #include <set>
 
+#include "host_pool.h"
 #include "plan.h"
 
 namespace qlg {
@@ -530,9 +531,17 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       }
   }
   pt.mark("s2-wc");
-  std::vector<double> Ch;
-  int64_t coff = 0;
-  for (Chan &c : ch) {
+  // Per channel, independently and on host threads: its outputs, their row
+  // descriptors and its combined operator.  The lists are then joined in
+  // channel order, so the plan is the same for any thread count.
+  struct ChanBuilt {
+    std::vector<RowDesc> orows;
+    std::vector<double> coef;
+  };
+  std::vector<ChanBuilt> built(ch.size());
+  parallel_for((int64_t)ch.size(), [&](int64_t ci) {
+    Chan &c = ch[(size_t)ci];
+    ChanBuilt &cb = built[(size_t)ci];
     // structurally reachable outputs that feed an existing R block and output
     // block; they depend on an input row's sectors only, not its components
     std::vector<std::array<int, 3>> trip;   // (w sector, s0 sector, s1 sector)
@@ -580,17 +589,9 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     }
     sort_unique(outs);
     c.out.swap(outs);
-    if (c.out.empty() || c.in.empty()) continue;
-    const int64_t Dap = L.ax[0].deg[c.ap], Db = psi.ax[bax].deg[c.b];
-    ChanTask t{};
-    t.c_off = coff;
-    t.kin = (int32_t)c.in.size();
-    t.kout = (int32_t)c.out.size();
-    t.P = (int32_t)(Dap * Db);
-    t.Db = (int32_t)Db;
-    t.in_rows = (int32_t)s2.rows.size();
-    for (const InRow &r : c.in) s2.rows.push_back(r.d);
-    t.out_rows = (int32_t)s2.rows.size();
+    if (c.out.empty() || c.in.empty()) return;
+    const int64_t Db = psi.ax[bax].deg[c.b];
+    cb.orows.reserve(c.out.size());
     for (const Key7 &o : c.out) {
       const int bp = o[6];
       Coords ob{};
@@ -610,11 +611,11 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
       RowDesc d;
       d.off = Y.base + row0 * Y.ld + cit->second + (int64_t)o[5] * Db;
       d.rs = Y.ld;
-      s2.rows.push_back(d);
+      cb.orows.push_back(d);
     }
     // combined operator rows (outputs) x columns (inputs)
     const int64_t K = (int64_t)c.in.size();
-    Ch.resize((coff + (int64_t)c.out.size() * K) * ed, 0.0);
+    cb.coef.assign((size_t)((int64_t)c.out.size() * K * ed), 0.0);
     for (size_t r = 0; r < c.out.size(); ++r) {
       const Key7 &o = c.out[r];
       const int64_t tg0 = so_t[0][o[0]] + o[1];
@@ -634,12 +635,30 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
           vr = Wc[i * ed];
           if (ed == 2) vi = Wc[i * ed + 1];
         }
-        const int64_t e = coff + (int64_t)r * K + q;
-        Ch[e * ed] = vr;
-        if (ed == 2) Ch[e * ed + 1] = vi;
+        const int64_t e = (int64_t)r * K + q;
+        cb.coef[e * ed] = vr;
+        if (ed == 2) cb.coef[e * ed + 1] = vi;
       }
     }
-    coff += (int64_t)c.out.size() * K;
+  });
+  std::vector<double> Ch;
+  int64_t coff = 0;
+  for (size_t ci = 0; ci < ch.size(); ++ci) {
+    const Chan &c = ch[ci];
+    if (c.out.empty() || c.in.empty()) continue;
+    const int64_t Dap = L.ax[0].deg[c.ap], Db = psi.ax[bax].deg[c.b];
+    ChanTask t{};
+    t.c_off = coff;
+    t.kin = (int32_t)c.in.size();
+    t.kout = (int32_t)c.out.size();
+    t.P = (int32_t)(Dap * Db);
+    t.Db = (int32_t)Db;
+    t.in_rows = (int32_t)s2.rows.size();
+    for (const InRow &r : c.in) s2.rows.push_back(r.d);
+    t.out_rows = (int32_t)s2.rows.size();
+    s2.rows.insert(s2.rows.end(), built[ci].orows.begin(), built[ci].orows.end());
+    Ch.insert(Ch.end(), built[ci].coef.begin(), built[ci].coef.end());
+    coff += (int64_t)c.out.size() * (int64_t)c.in.size();
     s2.chan.push_back(t);
     s2.flops += fl * double(t.kout) * double(t.kin) * double(t.P);
   }
