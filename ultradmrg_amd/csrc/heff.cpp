@@ -126,72 +126,98 @@ Plan *build_plan(Ctx *ctx, int nphys, const qlg_bsten *Ld, const qlg_bsten *cons
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
   };
+  // L blocks by bra sector a' (in block order), and the bra sectors in order
+  std::map<int, std::vector<int64_t>> L_by_bra;
+  for (size_t lb = 0; lb < L.blk.size(); ++lb) L_by_bra[L.blk[lb][0]].push_back((int64_t)lb);
+  std::vector<int> bras;
+  for (auto &kv : L_by_bra) bras.push_back(kv.first);
+  // The pass runs per bra sector a' on host threads: a' is the first
+  // coordinate of every intermediate and output block and no stage changes
+  // it, so the per-a' sorted sets, joined in a' order, are the sorted global
+  // set.  The FLOPs are summed per a' and then in a' order.
   auto symbolic = [&](const std::vector<Coords> &in, double *flops) {
-    std::vector<Coords> out;
-    double f = 0, f1 = 0, f4 = 0;
-    // stage T1: (a', w, s..., b)
-    std::vector<Coords> t1;
-    for (const Coords &c : in) {
-      auto it = L_by_ket.find(c[0]);
-      if (it == L_by_ket.end()) continue;
-      int64_t nfree = 1;
-      for (int i = 1; i < rk; ++i) nfree *= psi.ax[i].deg[c[i]];
-      for (int64_t lb : it->second) {
+    std::unordered_map<int, std::vector<int64_t>> in_by_ket;   // a -> entries of `in`
+    for (size_t i = 0; i < in.size(); ++i) in_by_ket[in[i][0]].push_back((int64_t)i);
+    struct Part {
+      std::vector<Coords> out;
+      double f1 = 0, f = 0, f4 = 0;
+    };
+    std::vector<Part> part(bras.size());
+    parallel_for((int64_t)bras.size(), [&](int64_t bi) {
+      Part &pr = part[(size_t)bi];
+      // stage T1: (a', w, s..., b)
+      std::vector<Coords> t1;
+      for (int64_t lb : L_by_bra.at(bras[(size_t)bi])) {
         const Coords &l = L.blk[lb];
-        Coords t{};
-        t[0] = l[0];
-        t[1] = l[1];
-        for (int i = 1; i < rk; ++i) t[1 + i] = c[i];
-        t1.push_back(t);
-        f1 += fl * double(L.ax[0].deg[l[0]] * L.ax[1].deg[l[1]]) * double(L.ax[2].deg[l[2]]) * double(nfree);
-      }
-    }
-    // MPO stages: carry (a', [s' done...], w_cur, [s remaining...], b)
-    // represent as Coords: [a', t_0..t_{k-1}, w, s_k..s_{n-1}, b]
-    sort_unique(t1);
-    std::vector<Coords> cur;  // k = 0: [a', w, s0, (s1), b]
-    cur.swap(t1);
-    for (int k = 0; k < nphys; ++k) {
-      std::vector<Coords> nxt;
-      for (const Coords &c : cur) {
-        const int w = c[1 + k], s = c[2 + k];
-        auto it = W_by_in[k].find({w, s});
-        if (it == W_by_in[k].end()) continue;
-        // free dims of the intermediate other than (w, s)
-        int64_t mfree = L.ax[0].deg[c[0]] * R.ax[2].deg[c[rk]];   // a' and b
-        for (int j = 0; j < k; ++j) mfree *= p.W[j].ax[2].deg[c[1 + j]];
-        for (int j = k + 1; j < nphys; ++j) mfree *= psi.ax[1 + j].deg[c[2 + j]];
-        const int64_t kdim = p.W[k].ax[0].deg[w] * p.W[k].ax[1].deg[s];
-        for (int64_t wb : it->second) {
-          const Coords &wc = p.W[k].blk[wb];
-          Coords t = c;
-          t[1 + k] = wc[2];   // s_out
-          t[2 + k] = wc[3];   // w right
-          nxt.push_back(t);
-          f += fl * double(mfree) * double(kdim) * double(p.W[k].ax[2].deg[wc[2]] * p.W[k].ax[3].deg[wc[3]]);
+        auto it = in_by_ket.find(l[2]);
+        if (it == in_by_ket.end()) continue;
+        for (int64_t ii : it->second) {
+          const Coords &c = in[(size_t)ii];
+          int64_t nfree = 1;
+          for (int i = 1; i < rk; ++i) nfree *= psi.ax[i].deg[c[i]];
+          Coords t{};
+          t[0] = l[0];
+          t[1] = l[1];
+          for (int i = 1; i < rk; ++i) t[1 + i] = c[i];
+          t1.push_back(t);
+          pr.f1 += fl * double(L.ax[0].deg[l[0]] * L.ax[1].deg[l[1]]) * double(L.ax[2].deg[l[2]]) * double(nfree);
         }
       }
-      sort_unique(nxt);
-      cur.swap(nxt);
-    }
-    // cur: [a', t..., z, b] ; contract (z, b) with R
-    for (const Coords &c : cur) {
-      const int z = c[1 + nphys], b = c[2 + nphys];
-      auto it = R_by_zb.find({z, b});
-      if (it == R_by_zb.end()) continue;
-      int64_t m = L.ax[0].deg[c[0]];
-      for (int j = 0; j < nphys; ++j) m *= p.W[j].ax[2].deg[c[1 + j]];
-      const int64_t kd = R.ax[1].deg[z] * R.ax[2].deg[b];
-      for (int64_t rb : it->second) {
-        Coords o{};
-        o[0] = c[0];
-        for (int j = 0; j < nphys; ++j) o[1 + j] = c[1 + j];
-        o[bax] = R.blk[rb][0];
-        out.push_back(o);
-        f4 += fl * double(m) * double(kd) * double(R.ax[0].deg[R.blk[rb][0]]);
+      // MPO stages: carry (a', [s' done...], w_cur, [s remaining...], b)
+      // represent as Coords: [a', t_0..t_{k-1}, w, s_k..s_{n-1}, b]
+      sort_unique(t1);
+      std::vector<Coords> cur;  // k = 0: [a', w, s0, (s1), b]
+      cur.swap(t1);
+      for (int k = 0; k < nphys; ++k) {
+        std::vector<Coords> nxt;
+        for (const Coords &c : cur) {
+          const int w = c[1 + k], s = c[2 + k];
+          auto it = W_by_in[k].find({w, s});
+          if (it == W_by_in[k].end()) continue;
+          // free dims of the intermediate other than (w, s)
+          int64_t mfree = L.ax[0].deg[c[0]] * R.ax[2].deg[c[rk]];   // a' and b
+          for (int j = 0; j < k; ++j) mfree *= p.W[j].ax[2].deg[c[1 + j]];
+          for (int j = k + 1; j < nphys; ++j) mfree *= psi.ax[1 + j].deg[c[2 + j]];
+          const int64_t kdim = p.W[k].ax[0].deg[w] * p.W[k].ax[1].deg[s];
+          for (int64_t wb : it->second) {
+            const Coords &wc = p.W[k].blk[wb];
+            Coords t = c;
+            t[1 + k] = wc[2];   // s_out
+            t[2 + k] = wc[3];   // w right
+            nxt.push_back(t);
+            pr.f += fl * double(mfree) * double(kdim) * double(p.W[k].ax[2].deg[wc[2]] * p.W[k].ax[3].deg[wc[3]]);
+          }
+        }
+        sort_unique(nxt);
+        cur.swap(nxt);
       }
+      // cur: [a', t..., z, b] ; contract (z, b) with R
+      for (const Coords &c : cur) {
+        const int z = c[1 + nphys], b = c[2 + nphys];
+        auto it = R_by_zb.find({z, b});
+        if (it == R_by_zb.end()) continue;
+        int64_t m = L.ax[0].deg[c[0]];
+        for (int j = 0; j < nphys; ++j) m *= p.W[j].ax[2].deg[c[1 + j]];
+        const int64_t kd = R.ax[1].deg[z] * R.ax[2].deg[b];
+        for (int64_t rb : it->second) {
+          Coords o{};
+          o[0] = c[0];
+          for (int j = 0; j < nphys; ++j) o[1 + j] = c[1 + j];
+          o[bax] = R.blk[rb][0];
+          pr.out.push_back(o);
+          pr.f4 += fl * double(m) * double(kd) * double(R.ax[0].deg[R.blk[rb][0]]);
+        }
+      }
+      sort_unique(pr.out);
+    });
+    std::vector<Coords> out;
+    double f = 0, f1 = 0, f4 = 0;
+    for (Part &pr : part) {
+      out.insert(out.end(), pr.out.begin(), pr.out.end());
+      f1 += pr.f1;
+      f += pr.f;
+      f4 += pr.f4;
     }
-    sort_unique(out);
     // [S1, S2 (+S3), S4] of the reference schedule
     flops[0] = f1;
     flops[1] = f;
