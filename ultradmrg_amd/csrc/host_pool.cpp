@@ -17,6 +17,16 @@ namespace qlg {
 
 namespace {
 
+// set while this thread runs items of a loop: a loop started from inside
+// one runs serially, without touching the pool's locks
+thread_local bool t_in_loop = false;
+
+struct InLoop {
+  bool prev;
+  InLoop() : prev(t_in_loop) { t_in_loop = true; }
+  ~InLoop() { t_in_loop = prev; }
+};
+
 struct Pool {
   std::mutex run_mu;                 // one loop at a time
   std::mutex mu;
@@ -35,6 +45,7 @@ struct Pool {
 
   // take items until none are left; record the first exception
   void drain(const std::function<void(int64_t)> &f, int64_t count) {
+    InLoop mark;
     for (int64_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
       try {
         f(i);
@@ -109,7 +120,7 @@ int host_workers() { return pool()->workers; }
 
 void parallel_for(int64_t n, const std::function<void(int64_t)> &f) {
   if (n <= 0) return;
-  Pool *p = n > 1 ? pool() : nullptr;
+  Pool *p = (n > 1 && !t_in_loop) ? pool() : nullptr;
   if (!p || p->workers == 0) {
     for (int64_t i = 0; i < n; ++i) f(i);
     return;
