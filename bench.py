@@ -598,7 +598,7 @@ def main():
     ap.add_argument("--no-chains", action="store_true")
     ap.add_argument("--no-tdvp", action="store_true")
     ap.add_argument("--no-chain", action="store_true", help="skip the L=100 Heisenberg ramp")
-    ap.add_argument("--tdvp-steps", type=int, default=3, help="TDVP steps timed per form (config 5 runs 40)")
+    ap.add_argument("--tdvp-steps", type=int, default=3, help="TDVP steps timed per form (the full D=1500 doublon-quench run is 40)")
     ap.add_argument("--sweep-profile", action="store_true",
                     help="only the Hubbard ramp and one D=2000 sweep per precision (for a profiler run)")
     a = ap.parse_args()

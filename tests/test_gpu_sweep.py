@@ -495,7 +495,7 @@ def test_dmrg_heisenberg_l40_stays_canonical():
     assert 40 * (0.25 - np.log(2)) < e < -0.43 * 40, e
 
 
-def test_config_c2_chain_vs_cpu_path():
+def test_random_field_chain_l100_vs_cpu_path():
     """Config 2's chain at reduced bond dimension, against the CPU path: an
     open spin-1/2 chain of L=100, Sz=0, with small random fields
     h_i ~ U(-0.5, 0.5) (they lift the uniform chain's SU(2) multiplets, so no
@@ -512,7 +512,7 @@ def test_config_c2_chain_vs_cpu_path():
     correct SVDs differ; the trajectories then part at the 1e-8 level."""
     from oracle.dmrg import SweepParams as OSweepParams, two_site_vmps
     from oracle.lanczos import LanczosParams as OLanczosParams
-    print("[case] config 2 chain L=100 D<=24 vs CPU path", flush=True)
+    print("[case] Heisenberg chain with random fields L=100 D<=24 vs CPU path", flush=True)
     fields = list(np.random.default_rng(1002).uniform(-0.5, 0.5, 100))
     m = o_heis(100, fields=fields)
     mpo = o_build_mpo(m)
