@@ -262,7 +262,7 @@ def captured_matvec(drv, site, modes, reps=20):
 
 def hubbard_sweep_comparison(split="bf16x6", alt=(), lx=16, ramp=(10, 100, 1000), D=2000, max_converge=12,
                              conv_tol=1e-8, nsweeps=2, with_matvec=True, dmrg_form=True):
-    """Config 3: the 4x16 Hubbard cylinder (t=1, U=8, 56 electrons, Sz=0, the
+    """The D=2000 sweep workload: the 4x16 Hubbard cylinder (t=1, U=8, 56 electrons, Sz=0, the
     reference benchmark's initial product state, benchmark/ultradmrg_hubbard2d.cpp:130-248;
     Lanczos 1e-9 / 100, truncation 1e-8) ramped through the benchmark's bond
     dimensions ``ramp`` = 10, 100, 1000 and then D=2000 (:237-248), at each D

@@ -175,7 +175,7 @@ def test_finite_dmrg_hubbard_2x2_kat():
     assert abs(e - (-2.828427124746)) < 1e-10
 
 
-def test_finite_dmrg_config1_matches_vmps_form():
+def test_finite_dmrg_heisenberg_l20_matches_vmps_form():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps,
     Lanczos(1e-10, 100): the DMRG form reaches the exact
     -8.682473334398967 (1e-10 relative) and agrees with the VMPS form update

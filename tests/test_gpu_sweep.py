@@ -279,7 +279,7 @@ def test_dmrg_kitaev_simple_kat(cplx):
     assert abs(drv.run() - (-1.0)) < 1e-12
 
 
-def test_dmrg_config1_heisenberg_l20():
+def test_dmrg_heisenberg_l20_d64():
     """Open Heisenberg chain L=20, Sz=0, seeded random MPS at D=64, 4 sweeps, Lanczos(1e-10, 100),
     truncation 1e-10: E0 = -8.682473334398967 (exact diagonalisation of the 184756-dim sector)."""
     print("[case] dmrg L=20 D=64", flush=True)
@@ -289,7 +289,7 @@ def test_dmrg_config1_heisenberg_l20():
                                                        verbose=False))
     e = drv.run()
     assert abs(e - (-8.682473334398967)) < 1e-10 * 8.68
-    print(f"[config1] E0 {e:.15f} sweep seconds {[round(s, 3) for s in drv.stats.sweep_seconds]}", flush=True)
+    print(f"[L=20 D=64] E0 {e:.15f} sweep seconds {[round(s, 3) for s in drv.stats.sweep_seconds]}", flush=True)
 
 
 def _rank_deficient(psi: BSTensor, r: int, seed: int) -> BSTensor:
@@ -496,7 +496,7 @@ def test_dmrg_heisenberg_l40_stays_canonical():
 
 
 def test_random_field_chain_l100_vs_cpu_path():
-    """Config 2's chain at reduced bond dimension, against the CPU path: an
+    """The L=100 Heisenberg chain of the D=1024 ramp, at reduced bond dimension, against the CPU path: an
     open spin-1/2 chain of L=100, Sz=0, with small random fields
     h_i ~ U(-0.5, 0.5) (they lift the uniform chain's SU(2) multiplets, so no
     cut falls inside a degenerate multiplet), the oracle's seeded random MPS
